@@ -1,0 +1,117 @@
+#ifndef REND_HIP_H
+#define REND_HIP_H
+
+/*
+ * Out-of-band C-ABI entry points of librend beyond the reference's rend.h.
+ * The reference has no counterpart for these: it returns void/0 everywhere
+ * (rend.h:149-162) and traces rays only from inside rend_render.  They give
+ * callers explicit ray buffers (the traversal parity and throughput contract),
+ * error reporting, the path-tracing render mode and batched multi-camera
+ * rendering.  All pointers named d_* are device (HBM) pointers; `stream` is a
+ * hipStream_t passed as void* (NULL = the library's own stream).
+ *
+ * Every function returns 0 on success and a negative REND_E* code on failure;
+ * rend_hip_lasterror()/rend_hip_errstr() give the last failure of the
+ * calling process.
+ */
+
+#include "rend.h"
+#include "rend_rays.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REND_OK          0
+#define REND_ENODEV     -1   /* no HIP device or HIP runtime error */
+#define REND_ENOMEM     -2   /* device or host allocation failed */
+#define REND_EINVAL     -3   /* invalid argument or scene */
+#define REND_ESTATE     -4   /* call order: scene not prepared/uploaded */
+#define REND_EOVERFLOW  -5   /* traversal stack or iteration limit hit */
+
+/* Reciprocal used by the ray/triangle test. */
+#define REND_RCP_HOST_TABLE  0   /* x86 RCPSS table: matches the reference bit for bit */
+#define REND_RCP_IEEE        1   /* correctly rounded 1/det */
+
+int          rend_hip_lasterror(void);
+const char  *rend_hip_errstr(void);
+
+/* Device properties (compute units, name) and the table's capture CPU. */
+int          rend_hip_devinfo(int *cus, char *name, unsigned int namelen);
+const char  *rend_hip_rcp_cpu(void);
+
+/* Select the reciprocal mode for later traces (default HOST_TABLE). */
+int          rend_hip_set_rcp(int mode);
+
+/* Re-upload the scene of rd (normally done by rend_prepstatic and
+ * rend_prepdynamic). */
+int          rend_hip_upload(struct rdata *rd);
+
+/* Closest hit for n rays of d_rays into d_hits (device buffers). Reference
+ * semantics: intersect_tlas (rend.c:1976-2155) per ray, starting t = tmax.
+ * stats may be NULL; when given, visit counters are accumulated into it
+ * (slower, counting launch). */
+int          rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
+                            struct rhit *d_hits, unsigned int n,
+                            void *stream, struct rtrace_stats *stats);
+
+/* Any hit within (0, tmax): d_occ[i] = 1 when occluded, else 0.  Reference
+ * semantics: intersect_any_tlas (rend.c:2883-2998). */
+int          rend_hip_occluded(struct rdata *rd, const struct rray *d_rays,
+                               unsigned char *d_occ, unsigned int n,
+                               void *stream);
+
+/* Host-buffer conveniences: copy in, trace, copy out (synchronous). */
+int          rend_hip_trace_host(struct rdata *rd, const struct rray *rays,
+                                 struct rhit *hits, unsigned int n,
+                                 struct rtrace_stats *stats);
+int          rend_hip_occluded_host(struct rdata *rd, const struct rray *rays,
+                                    unsigned char *occ, unsigned int n);
+
+/* Traversal launch shape: persistent blocks per CU (0 = occupancy limit)
+ * and the number of idle lanes that triggers a wave refill (1..64). */
+int          rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill);
+
+/* Builder hazards (bit set) met while building rd's trees; nonzero means the
+ * reference's own build would be undefined or degenerate for this scene. */
+unsigned int rend_hip_hazards(struct rdata *rd);
+
+/* Bytes of the device BVH arena (BLASes + TLAS), -1 before upload. */
+long long    rend_hip_arena_bytes(struct rdata *rd);
+
+/* Frame rendering.  REND_MODE_NORMALS is the reference's active shading
+ * (rend.c:3404-3563); REND_MODE_PATHTRACE is the wavefront path tracer
+ * (trace3 semantics, rend.c:3205-3291, generalised). */
+#define REND_MODE_NORMALS    0
+#define REND_MODE_PATHTRACE  1
+int          rend_hip_set_mode(int mode);
+
+/* Renders one frame of rd (what rend_render does for the claiming thread)
+ * and reports the rays traced; returns an error code instead of 0. */
+int          rend_hip_render_frame(struct rdata *rd, unsigned int *rays);
+
+/* Copies the device accumulator into rd->acc (kept on the device between
+ * frames; rd->buf is refreshed after every frame). */
+int          rend_hip_download_acc(struct rdata *rd);
+
+/* The library's HIP stream (hipStream_t as void*). */
+void        *rend_hip_stream(void);
+
+/* Device memory helpers for callers without another HIP binding. */
+void        *rend_hip_malloc(unsigned long long bytes);
+int          rend_hip_free(void *d);
+int          rend_hip_memcpy_htod(void *d, const void *h, unsigned long long bytes);
+int          rend_hip_memcpy_dtoh(void *h, const void *d, unsigned long long bytes);
+int          rend_hip_sync(void);
+
+/* Event timing on a stream: returns milliseconds between two marks. */
+void        *rend_hip_event_create(void);
+int          rend_hip_event_record(void *ev, void *stream);
+float        rend_hip_event_elapsed(void *start, void *stop);
+int          rend_hip_event_destroy(void *ev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

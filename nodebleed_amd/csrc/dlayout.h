@@ -1,0 +1,71 @@
+/*
+ * Device (HBM) data layout of a scene, shared by the host C code that
+ * repacks and uploads it and the HIP kernels that read it.  Plain C types
+ * only.
+ *
+ * Arena: one byte-addressed buffer holding every BLAS and the TLAS.  Within
+ * one tree, child words keep the reference's encoding (rend.h:13, 24-27):
+ * NODE_LEAF | offset of a dleaf (BLAS), NODE_LEAF | instance id (TLAS), or
+ * the offset of a child dnode, relative to that tree's root.  Nodes sit at
+ * 256-byte aligned offsets so a node is exactly two 128-byte cache lines;
+ * leaves follow all nodes of their tree.
+ */
+#ifndef DLAYOUT_H
+#define DLAYOUT_H
+
+#include <stdint.h>
+
+#define DNODE_LEAF  0x80000000u
+
+/* 256 bytes.  Bounds and child words are the reference's bnode8 values;
+ * inv[o] replaces perm8: bits [3c, 3c+3) give child c's traversal rank for
+ * ray-direction octant o (rank 7 = nearest, visited first). */
+struct dnode {
+	float     minx[8];
+	float     maxx[8];
+	float     miny[8];
+	float     maxy[8];
+	float     minz[8];
+	float     maxz[8];
+	uint32_t  child[8];
+	uint32_t  inv[8];
+};
+
+/* 192 bytes, identical to the reference's leaf4 (rend.h:32-45). */
+struct dleaf {
+	float     v0x[4], v0y[4], v0z[4];
+	float     e0x[4], e0y[4], e0z[4];
+	float     e1x[4], e1y[4], e1z[4];
+	uint32_t  id[4];
+	uint32_t  tricnt;
+	uint32_t  pad[7];
+};
+
+/* 64 bytes: inverse transform rows (rinst.globinv), BLAS root byte offset in
+ * the arena, first triangle, flags. */
+struct dinst {
+	float     inv[12];
+	uint64_t  blas;
+	uint32_t  triofs;
+	uint32_t  flags;
+};
+
+/* Camera constants of one view, taken from the caller's rcam (rend.h:89-117,
+ * filled by upd_rcam, main.c:159-195). */
+struct CamArgs {
+	float  eye[3], ri[3], up[3], fwd[3];
+	float  rw, rh, aspect, fdist, fovfdist, focrad;
+	int    dof;
+};
+
+/* Traversal stack: entries kept in LDS per lane, deeper ones spill to a
+ * global buffer of (REND_STACK_MAX - REND_LDS_STACK) entries per thread. */
+#define REND_STACK_MAX   64   /* reference bound, rend.c:834 assert */
+#define REND_LDS_STACK   16
+#define REND_TRACE_BLOCK 256
+
+/* Error bits written by kernels into the device status word. */
+#define RK_ERR_STACK  1u
+#define RK_ERR_ITER   2u
+
+#endif

@@ -1,0 +1,242 @@
+/*
+ * rend_render on the GPU (rend.c:3603-3708 protocol, main.c:386-427).
+ *
+ * The reference runs rend_render on every host core; each thread claims
+ * 64x32 blocks with an atomic counter until blkcnt = floor(w/bx)*floor(h/by)
+ * blocks are taken, so rows and columns past the last full block are never
+ * rendered.  Here the first caller of a frame claims all blocks at once
+ * (fetch_add(&blknum, blkcnt) returns 0 only for it), renders the block grid
+ * on the GPU and copies the ARGB buffer to rd->buf; every other caller sees
+ * blknum >= blkcnt and returns 0.  rays is incremented by the number of rays
+ * traced, as the reference does.
+ */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+
+#include <stdlib.h>
+#include <string.h>
+
+#include "rend_int.h"
+#include "rend_hip.h"
+#include "librend.h"
+#include "../dlayout.h"
+
+int rk_camrays(const struct CamArgs *c, unsigned rw, unsigned rh,
+               unsigned spp, unsigned seed, unsigned frame, void *out,
+               void *stream);
+int rk_shade_normals(const void *hits, const void *insts, const void *nrms,
+                     const void *mtls, const float bg[3], unsigned rw,
+                     unsigned rh, unsigned width, unsigned spp,
+                     float invsamples, float *acc, unsigned *buf,
+                     void *stream);
+int rk_fill_f32(float *p, unsigned long long n, float v, void *stream);
+
+struct rl_frame {
+	void    *rays, *hits;
+	size_t   cap;        /* rays */
+	unsigned frame;      /* frames rendered since the last clear */
+};
+
+void rl_frame_release(struct rl_scene *s)
+{
+	if (!s->frame)
+		return;
+	if (s->frame->rays)
+		hipFree(s->frame->rays);
+	if (s->frame->hits)
+		hipFree(s->frame->hits);
+	free(s->frame);
+	s->frame = NULL;
+}
+
+void rl_camargs(const struct rcam *c, struct CamArgs *a)
+{
+	const float *f;
+#define L0(v) (f = (const float *)&(v), f[0])
+	a->eye[0] = L0(c->eyex8);
+	a->eye[1] = L0(c->eyey8);
+	a->eye[2] = L0(c->eyez8);
+	a->ri[0] = L0(c->rix8);
+	a->ri[1] = L0(c->riy8);
+	a->ri[2] = L0(c->riz8);
+	a->up[0] = L0(c->upx8);
+	a->up[1] = L0(c->upy8);
+	a->up[2] = L0(c->upz8);
+	a->fwd[0] = L0(c->fwdx8);
+	a->fwd[1] = L0(c->fwdy8);
+	a->fwd[2] = L0(c->fwdz8);
+	a->rw = L0(c->rw8);
+	a->rh = L0(c->rh8);
+	a->aspect = L0(c->aspect8);
+	a->fdist = L0(c->fdist8);
+	a->fovfdist = L0(c->fovfdist8);
+	a->focrad = L0(c->focrad8);
+#undef L0
+	a->dof = c->tanfangle > 0.0f;
+}
+
+/* Device accumulator and colour buffer sized for the current resolution. */
+int rl_acc_ready(struct rl_scene *s, hipStream_t st)
+{
+	struct rdata *rd = s->rd;
+	size_t npx = (size_t)rd->width * rd->height;
+	hipError_t e;
+	if (npx > s->acc_cap) {
+		if (s->acc)
+			hipFree(s->acc);
+		if (s->buf)
+			hipFree(s->buf);
+		s->acc = NULL;
+		s->buf = NULL;
+		s->acc_cap = 0;
+		if ((e = hipMalloc((void **)&s->acc, npx * 3 * sizeof(float)))
+		      != hipSuccess ||
+		    (e = hipMalloc((void **)&s->buf, npx * sizeof(unsigned)))
+		      != hipSuccess)
+			return rl_hipfail(e, "accumulator");
+		s->acc_cap = npx;
+		s->acc_dirty = 1;
+	}
+	if (s->acc_dirty) {
+		int r = rk_fill_f32(s->acc, npx * 3, 0.0f, st);
+		if (r)
+			return rl_hipfail(r, "accumulator clear");
+		s->acc_dirty = 0;
+		if (s->frame)
+			s->frame->frame = 0;
+	}
+	return 0;
+}
+
+static int frame_buffers(struct rl_scene *s, size_t nrays)
+{
+	if (!s->frame) {
+		s->frame = calloc(1, sizeof(*s->frame));
+		if (!s->frame)
+			return rl_fail(REND_ENOMEM, "frame state");
+	}
+	struct rl_frame *f = s->frame;
+	if (nrays <= f->cap)
+		return 0;
+	if (f->rays)
+		hipFree(f->rays);
+	if (f->hits)
+		hipFree(f->hits);
+	f->rays = f->hits = NULL;
+	f->cap = 0;
+	hipError_t e;
+	if ((e = hipMalloc(&f->rays, nrays * sizeof(struct rray))) != hipSuccess ||
+	    (e = hipMalloc(&f->hits, nrays * sizeof(struct rhit))) != hipSuccess)
+		return rl_hipfail(e, "frame ray buffers");
+	f->cap = nrays;
+	return 0;
+}
+
+/* Normal-tint frame (the reference's active rend_render). */
+static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
+                          unsigned *rays_out)
+{
+	struct rdata *rd = s->rd;
+	hipStream_t st = G.stream;
+	int r = rl_acc_ready(s, st);
+	if (r)
+		return r;
+	unsigned spp = rd->spp ? rd->spp : 1;
+	size_t nrays = (size_t)rw * rh * spp;
+	if ((r = frame_buffers(s, nrays)))
+		return r;
+	struct CamArgs ca;
+	rl_camargs(&rd->cam, &ca);
+	struct rl_frame *f = s->frame;
+	if ((r = rk_camrays(&ca, rw, rh, spp, 0x2545f491u, rd->samples, f->rays,
+	  st)))
+		return rl_hipfail(r, "camera rays");
+	if ((r = rl_trace_async(s, f->rays, f->hits, NULL, (unsigned)nrays, 1, 0,
+	  st)))
+		return r;
+	float bg[3] = {rd->bgcol.x, rd->bgcol.y, rd->bgcol.z};
+	float inv = 1.0f / (float)(rd->spp + rd->samples);
+	if ((r = rk_shade_normals(f->hits, s->insts, s->nrms, s->mtls, bg, rw, rh,
+	  rd->width, spp, inv, s->acc, s->buf, st)))
+		return rl_hipfail(r, "shade");
+	f->frame++;
+	*rays_out = (unsigned)nrays;
+	return 0;
+}
+
+int rend_render(void *d)
+{
+	struct rdata *rd = d;
+	unsigned bx = rl_blkszx(), by = rl_blkszy();
+	unsigned bcx = rd->width / bx, bcy = rd->height / by;
+	int blkcnt = (int)(bcx * bcy);
+	if (blkcnt <= 0)
+		return 0;
+	int first = __atomic_fetch_add(&rd->blknum, blkcnt, __ATOMIC_SEQ_CST);
+	if (first != 0)
+		return 0;
+
+	unsigned rays = 0;
+	int r = rend_hip_render_frame(rd, &rays);
+	if (r)
+		return 0;
+	__atomic_fetch_add(&rd->rays, rays, __ATOMIC_SEQ_CST);
+	return 0;
+}
+
+static int g_mode = REND_MODE_NORMALS;
+
+int rend_hip_set_mode(int mode)
+{
+	if (mode != REND_MODE_NORMALS && mode != REND_MODE_PATHTRACE)
+		return rl_fail(REND_EINVAL, "render mode %d", mode);
+	g_mode = mode;
+	return 0;
+}
+
+int rend_hip_render_frame(struct rdata *rd, unsigned *rays)
+{
+	struct rl_scene *s = rl_scene(rd);
+	int r = rl_device();
+	if (r)
+		return r;
+	if (!s || !s->tlas_built)
+		return rl_fail(REND_ESTATE, "scene not prepared");
+	if ((r = rl_sync_scene(s)))
+		return r;
+	unsigned bx = rl_blkszx(), by = rl_blkszy();
+	unsigned rw = (rd->width / bx) * bx, rh = (rd->height / by) * by;
+	unsigned n = 0;
+	if (g_mode == REND_MODE_PATHTRACE)
+		r = rl_render_pt(s, rw, rh, &n);
+	else
+		r = render_normals(s, rw, rh, &n);
+	if (r)
+		return r;
+	hipError_t e = hipStreamSynchronize(G.stream);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "frame");
+	r = rl_check_errors(G.stream);
+	if (r)
+		return r;
+	if (rd->buf) {
+		/* rd->buf has pitch = width (rend.c:3571). */
+		e = hipMemcpy(rd->buf, s->buf, (size_t)rd->width * rd->height *
+		  sizeof(unsigned), hipMemcpyDeviceToHost);
+		if (e != hipSuccess)
+			return rl_hipfail(e, "colour buffer download");
+	}
+	if (rays)
+		*rays = n;
+	return 0;
+}
+
+int rend_hip_download_acc(struct rdata *rd)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->acc || !rd->acc)
+		return rl_fail(REND_ESTATE, "no accumulator");
+	hipError_t e = hipMemcpy(rd->acc, s->acc, (size_t)rd->width * rd->height *
+	  3 * sizeof(float), hipMemcpyDeviceToHost);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "accumulator download");
+}

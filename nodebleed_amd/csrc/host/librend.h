@@ -1,0 +1,87 @@
+/*
+ * Internal state of the librend host library: the process-global device
+ * context and the per-rdata device mirror.  Not part of the public ABI.
+ */
+#ifndef LIBREND_H
+#define LIBREND_H
+
+#include <hip/hip_runtime_api.h>
+
+#include "rend.h"
+#include "../dlayout.h"
+
+#define RL_CTL_SLOTS  64  /* ring of (work counter, error word) pairs */
+
+struct rl_global {
+	int                 ready;
+	hipStream_t         stream;
+	int                 cus;
+	char                name[256];
+	int                 rcp_ieee;
+	unsigned            refill;      /* idle lanes that trigger a wave refill */
+	unsigned            max_iter;    /* traversal step limit per ray */
+	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
+	unsigned           *ctl;         /* RL_CTL_SLOTS x {counter, error} */
+	unsigned            ctl_next;
+	unsigned            last_slot;
+	unsigned long long *stats;       /* 4 counters of the counting variant */
+	void               *spill;       /* traversal stack spill area */
+	size_t              spill_bytes;
+};
+
+struct rl_frame;  /* frame.c */
+
+struct rl_scene {
+	struct rdata        *rd;
+	unsigned             maxmtls, maxtris, maxinsts;
+	unsigned char       *arena;      /* device: BLASes then TLAS */
+	size_t               arena_cap, arena_used, blas_bytes, tlas_bytes;
+	unsigned long long   tlas;       /* byte offset of the TLAS root */
+	unsigned             nblas;
+	unsigned            *blas_triofs;
+	unsigned long long  *blas_dev;   /* byte offset of each BLAS root */
+	void                *insts;      /* device dinst[] */
+	size_t               insts_cap;
+	unsigned             ninsts;
+	void                *nrms, *mtls;
+	size_t               nrms_cap, mtls_cap;
+	float               *acc;        /* device accumulator, 3 floats/pixel */
+	unsigned            *buf;        /* device ARGB8888 */
+	size_t               acc_cap;
+	int                  acc_dirty;
+	int                  static_ok, dynamic_ok;
+	int                  static_dirty, dynamic_dirty, tlas_built;
+	unsigned             hazards;
+	struct rl_frame     *frame;
+};
+
+extern struct rl_global G;
+
+int              rl_fail(int code, const char *fmt, ...);
+int              rl_hipfail(int herr, const char *what);
+int              rl_device(void);
+struct rl_scene *rl_scene(const struct rdata *rd);
+int              rl_trace_grid(int closest, int count, unsigned *grid);
+int              rl_trace_async(struct rl_scene *s, const void *d_rays,
+                                void *d_hits, void *d_occ, unsigned n,
+                                int closest, int count, hipStream_t st);
+int              rl_check_errors(hipStream_t st);
+unsigned         rl_blkszx(void);
+unsigned         rl_blkszy(void);
+void             rl_frame_release(struct rl_scene *s);
+int              rl_sync_scene(struct rl_scene *s);
+int              rl_render_pt(struct rl_scene *s, unsigned rw, unsigned rh,
+                              unsigned *rays);
+int              rl_acc_ready(struct rl_scene *s, hipStream_t st);
+void             rl_camargs(const struct rcam *c, struct CamArgs *a);
+
+/* Kernel C-ABI (kern/ HIP sources). */
+int rk_trace_occupancy(int closest, int count, int rcp_ieee);
+int rk_trace_launch(const void *arena, unsigned long long tlas,
+                    const void *insts, const void *rays, void *hits,
+                    void *occ, unsigned int n, unsigned int *ctl,
+                    void *spill, unsigned long long *stats, int closest,
+                    int count, int rcp_ieee, unsigned int grid,
+                    unsigned int refill, unsigned int max_iter, void *stream);
+
+#endif

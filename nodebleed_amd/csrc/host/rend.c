@@ -1,0 +1,884 @@
+/*
+ * librend host library (C11): the reference renderer's API (rend.h:149-162)
+ * over one HIP device, plus the ray-buffer entry points of rend_hip.h.
+ *
+ * Ownership follows the reference (rend.c:3000-3107): rend_init allocates
+ * the host arrays the caller fills (mtls, tris, nrms, insts, aabbs) and the
+ * host BVH arena; the renderer owns acc, the application owns buf.  The HIP
+ * layer owns the device mirrors: rend_prepstatic builds every BLAS on the
+ * host, repacks it into the device layout (dlayout.h) and uploads it with the
+ * normals and materials; rend_prepdynamic builds the TLAS and uploads it with
+ * the instance records.
+ */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rend_int.h"
+#include "rend_hip.h"
+#include "librend.h"
+#include "../dlayout.h"
+#define RCP_TABLE_NO_HOST_ARRAY
+#include "../rcp_table.h"
+
+/* ---- errors -------------------------------------------------------------- */
+
+static int g_err;
+static char g_errmsg[256];
+
+int rl_fail(int code, const char *fmt, ...)
+{
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_errmsg, sizeof(g_errmsg), fmt, ap);
+	va_end(ap);
+	g_err = code;
+	return code;
+}
+
+int rl_hipfail(int herr, const char *what)
+{
+	return rl_fail(REND_ENODEV, "%s: %s", what,
+	  hipGetErrorString((hipError_t)herr));
+}
+
+int rend_hip_lasterror(void) { return g_err; }
+const char *rend_hip_errstr(void) { return g_errmsg; }
+const char *rend_hip_rcp_cpu(void) { return RCP_TABLE_CPU; }
+
+/* ---- process-global device state ----------------------------------------- */
+
+struct rl_global G;
+static pthread_mutex_t g_lock = PTHREAD_MUTEX_INITIALIZER;
+
+int rl_device(void)
+{
+	if (G.ready)
+		return 0;
+	pthread_mutex_lock(&g_lock);
+	if (G.ready) {
+		pthread_mutex_unlock(&g_lock);
+		return 0;
+	}
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess || n < 1) {
+		pthread_mutex_unlock(&g_lock);
+		return rl_fail(REND_ENODEV, "no HIP device (%s)",
+		  hipGetErrorString(e));
+	}
+	hipDeviceProp_t p;
+	if ((e = hipSetDevice(0)) != hipSuccess ||
+	    (e = hipGetDeviceProperties(&p, 0)) != hipSuccess ||
+	    (e = hipStreamCreateWithFlags(&G.stream, hipStreamNonBlocking))
+	      != hipSuccess) {
+		pthread_mutex_unlock(&g_lock);
+		return rl_hipfail(e, "device init");
+	}
+	G.cus = p.multiProcessorCount;
+	snprintf(G.name, sizeof(G.name), "%s", p.gcnArchName);
+	G.refill = 16;
+	G.max_iter = 1u << 22;
+	G.grid_mult = 0;
+	if ((e = hipMalloc((void **)&G.ctl, RL_CTL_SLOTS * 2 * sizeof(unsigned)))
+	      != hipSuccess ||
+	    (e = hipMemset(G.ctl, 0, RL_CTL_SLOTS * 2 * sizeof(unsigned)))
+	      != hipSuccess ||
+	    (e = hipMalloc((void **)&G.stats, 4 * sizeof(unsigned long long)))
+	      != hipSuccess) {
+		pthread_mutex_unlock(&g_lock);
+		return rl_hipfail(e, "control words");
+	}
+	G.ready = 1;
+	pthread_mutex_unlock(&g_lock);
+	return 0;
+}
+
+int rend_hip_devinfo(int *cus, char *name, unsigned int namelen)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	if (cus)
+		*cus = G.cus;
+	if (name && namelen)
+		snprintf(name, namelen, "%s", G.name);
+	return 0;
+}
+
+int rend_hip_set_rcp(int mode)
+{
+	if (mode != REND_RCP_HOST_TABLE && mode != REND_RCP_IEEE)
+		return rl_fail(REND_EINVAL, "rcp mode %d", mode);
+	G.rcp_ieee = mode == REND_RCP_IEEE;
+	return 0;
+}
+
+int rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill)
+{
+	if (refill < 1 || refill > 64)
+		return rl_fail(REND_EINVAL, "refill %u", refill);
+	G.grid_mult = grid_mult;
+	G.refill = refill;
+	return 0;
+}
+
+/* Persistent grid for a trace variant and the matching spill area. */
+int rl_trace_grid(int closest, int count, unsigned *grid)
+{
+	int occ = rk_trace_occupancy(closest, count, G.rcp_ieee);
+	if (occ < 1)
+		return rl_fail(REND_ENODEV, "trace kernel occupancy query failed");
+	unsigned mult = G.grid_mult ? G.grid_mult : (unsigned)occ;
+	unsigned g = (unsigned)G.cus * mult;
+	size_t need = (size_t)g * REND_TRACE_BLOCK *
+	  (REND_STACK_MAX - REND_LDS_STACK) * 8u;
+	if (need > G.spill_bytes) {
+		if (G.spill)
+			hipFree(G.spill);
+		G.spill = NULL;
+		G.spill_bytes = 0;
+		hipError_t e = hipMalloc(&G.spill, need);
+		if (e != hipSuccess)
+			return rl_hipfail(e, "trace spill area");
+		G.spill_bytes = need;
+	}
+	*grid = g;
+	return 0;
+}
+
+/* ---- scene registry -------------------------------------------------------- */
+
+#define MAX_SCENES 64
+static struct rl_scene *g_scenes[MAX_SCENES];
+
+struct rl_scene *rl_scene(const struct rdata *rd)
+{
+	for (int i = 0; i < MAX_SCENES; i++)
+		if (g_scenes[i] && g_scenes[i]->rd == rd)
+			return g_scenes[i];
+	return NULL;
+}
+
+static struct rl_scene *scene_add(struct rdata *rd)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (s)
+		return s;
+	pthread_mutex_lock(&g_lock);
+	for (int i = 0; i < MAX_SCENES; i++)
+		if (!g_scenes[i]) {
+			s = calloc(1, sizeof(*s));
+			if (s) {
+				s->rd = rd;
+				g_scenes[i] = s;
+			}
+			break;
+		}
+	pthread_mutex_unlock(&g_lock);
+	return s;
+}
+
+static void dfree(void **p)
+{
+	if (*p)
+		hipFree(*p);
+	*p = NULL;
+}
+
+static void scene_del(struct rdata *rd)
+{
+	for (int i = 0; i < MAX_SCENES; i++)
+		if (g_scenes[i] && g_scenes[i]->rd == rd) {
+			struct rl_scene *s = g_scenes[i];
+			dfree((void **)&s->arena);
+			dfree((void **)&s->insts);
+			dfree((void **)&s->nrms);
+			dfree((void **)&s->mtls);
+			dfree((void **)&s->acc);
+			dfree((void **)&s->buf);
+			rl_frame_release(s);
+			free(s->blas_triofs);
+			free(s->blas_dev);
+			free(s);
+			g_scenes[i] = NULL;
+		}
+}
+
+/* ---- reference API: statics ----------------------------------------------- */
+
+static unsigned g_blkszx = 64, g_blkszy = 32;
+
+void rend_staticinit(unsigned int bszx, unsigned int bszy)
+{
+	g_blkszx = bszx ? bszx : 1;
+	g_blkszy = bszy ? bszy : 1;
+}
+
+void rend_staticrelease(void)
+{
+}
+
+unsigned rl_blkszx(void) { return g_blkszx; }
+unsigned rl_blkszy(void) { return g_blkszy; }
+
+/* rend.c:171-187: left-to-right, separately rounded. */
+void mulpos(float *ox, float *oy, float *oz, float m[16], float x, float y,
+            float z)
+{
+	*ox = m[0] * x + m[1] * y + m[2] * z + m[3];
+	*oy = m[4] * x + m[5] * y + m[6] * z + m[7];
+	*oz = m[8] * x + m[9] * y + m[10] * z + m[11];
+}
+
+void muldir(float *ox, float *oy, float *oz, float m[16], float x, float y,
+            float z)
+{
+	*ox = m[0] * x + m[1] * y + m[2] * z;
+	*oy = m[4] * x + m[5] * y + m[6] * z;
+	*oz = m[8] * x + m[9] * y + m[10] * z;
+}
+
+/* ---- reference API: lifecycle --------------------------------------------- */
+
+static void *zalloc64(size_t bytes)
+{
+	size_t sz = (bytes + 63) & ~(size_t)63;
+	void *p = aligned_alloc(64, sz ? sz : 64);
+	return p;
+}
+
+void rend_init(struct rdata *rd, unsigned int maxmtls, unsigned int maxtris,
+               unsigned int maxinsts, unsigned int spp)
+{
+	rd->mtls = zalloc64((size_t)maxmtls * sizeof(*rd->mtls));
+	rd->tris = zalloc64((size_t)maxtris * sizeof(*rd->tris));
+	rd->nrms = zalloc64((size_t)maxtris * sizeof(*rd->nrms));
+	rd->insts = zalloc64((size_t)maxinsts * sizeof(*rd->insts));
+	rd->aabbs = zalloc64((size_t)maxinsts * sizeof(*rd->aabbs));
+	size_t nb = 2 * ((size_t)maxtris + maxinsts) * sizeof(*rd->bnodes);
+	rd->bnodes = zalloc64(nb);
+	if (rd->bnodes)
+		memset(rd->bnodes, 0, nb);
+	rd->tlasofs = 2 * maxtris;
+	rd->spp = spp;
+	if (!rd->mtls || !rd->tris || !rd->nrms || !rd->insts || !rd->aabbs ||
+	    !rd->bnodes) {
+		rl_fail(REND_ENOMEM, "rend_init: host allocation failed");
+		return;
+	}
+	struct rl_scene *s = scene_add(rd);
+	if (!s) {
+		rl_fail(REND_ENOMEM, "rend_init: scene registry full");
+		return;
+	}
+	s->maxmtls = maxmtls;
+	s->maxtris = maxtris;
+	s->maxinsts = maxinsts;
+}
+
+void rend_release(struct rdata *rd)
+{
+	scene_del(rd);
+	free(rd->acc);
+	free(rd->bnodes);
+	free(rd->aabbs);
+	free(rd->insts);
+	free(rd->nrms);
+	free(rd->tris);
+	free(rd->mtls);
+	rd->acc = NULL;
+	rd->bnodes = NULL;
+	rd->aabbs = NULL;
+	rd->insts = NULL;
+	rd->nrms = NULL;
+	rd->tris = NULL;
+	rd->mtls = NULL;
+}
+
+/* ---- BVH builds ------------------------------------------------------------ */
+
+static int blas_find(const struct rl_scene *s, unsigned triofs)
+{
+	for (unsigned i = 0; i < s->nblas; i++)
+		if (s->blas_triofs[i] == triofs)
+			return (int)i;
+	return -1;
+}
+
+/* Ensures the device arena holds at least `bytes`, keeping the first `keep`
+ * bytes. */
+static int arena_reserve(struct rl_scene *s, size_t bytes, size_t keep)
+{
+	if (bytes <= s->arena_cap)
+		return 0;
+	size_t cap = bytes + bytes / 8 + 4096;
+	unsigned char *na = NULL;
+	hipError_t e = hipMalloc((void **)&na, cap);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "device arena");
+	if (keep && s->arena) {
+		e = hipMemcpy(na, s->arena, keep, hipMemcpyDeviceToDevice);
+		if (e != hipSuccess) {
+			hipFree(na);
+			return rl_hipfail(e, "device arena copy");
+		}
+	}
+	dfree((void **)&s->arena);
+	s->arena = na;
+	s->arena_cap = cap;
+	return 0;
+}
+
+static int upload_static(struct rdata *rd, struct rl_scene *s)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	/* Distinct BLAS roots referenced by instances. */
+	free(s->blas_triofs);
+	free(s->blas_dev);
+	s->blas_triofs = malloc(((size_t)rd->instcnt + 1) * sizeof(unsigned));
+	s->blas_dev = malloc(((size_t)rd->instcnt + 1) * sizeof(unsigned long long));
+	s->nblas = 0;
+	if (!s->blas_triofs || !s->blas_dev)
+		return rl_fail(REND_ENOMEM, "blas map");
+
+	size_t total = 0;
+	long long *sizes = malloc(((size_t)rd->instcnt + 1) * sizeof(long long));
+	if (!sizes)
+		return rl_fail(REND_ENOMEM, "blas sizes");
+	for (unsigned j = 0; j < rd->instcnt; j++) {
+		const struct rinst *ri = &rd->insts[j];
+		if (blas_find(s, ri->triofs) >= 0)
+			continue;
+		const struct bnode8 *root = &rd->bnodes[(size_t)ri->triofs << 1];
+		size_t limit = 2 * (size_t)ri->tricnt * sizeof(struct bnode8);
+		if (limit < sizeof(struct bnode8))
+			limit = sizeof(struct bnode8);
+		long long sz = bvh8_repack_size(root, 1, limit, NULL, NULL);
+		if (sz < 0) {
+			free(sizes);
+			return rl_fail(REND_EINVAL, "malformed BLAS at triofs %u",
+			  ri->triofs);
+		}
+		s->blas_triofs[s->nblas] = ri->triofs;
+		s->blas_dev[s->nblas] = total;
+		sizes[s->nblas] = sz;
+		s->nblas++;
+		total += ((size_t)sz + 255) & ~(size_t)255;
+	}
+	size_t tlas_res = 2 * (size_t)(s->maxinsts ? s->maxinsts : 1) *
+	  sizeof(struct bnode8);
+	r = arena_reserve(s, total + tlas_res, 0);
+	if (r) {
+		free(sizes);
+		return r;
+	}
+	unsigned char *stage = NULL;
+	size_t stagecap = 0;
+	for (unsigned b = 0; b < s->nblas; b++) {
+		unsigned triofs = s->blas_triofs[b];
+		const struct bnode8 *root = &rd->bnodes[(size_t)triofs << 1];
+		unsigned tricnt = 0;
+		for (unsigned j = 0; j < rd->instcnt; j++)
+			if (rd->insts[j].triofs == triofs) {
+				tricnt = rd->insts[j].tricnt;
+				break;
+			}
+		size_t limit = 2 * (size_t)tricnt * sizeof(struct bnode8);
+		if (limit < sizeof(struct bnode8))
+			limit = sizeof(struct bnode8);
+		if ((size_t)sizes[b] > stagecap) {
+			free(stage);
+			stagecap = (size_t)sizes[b];
+			stage = aligned_alloc(64, (stagecap + 63) & ~(size_t)63);
+			if (!stage) {
+				free(sizes);
+				return rl_fail(REND_ENOMEM, "repack staging");
+			}
+		}
+		if (bvh8_repack(stage, stagecap, root, 1, limit) != sizes[b]) {
+			free(stage);
+			free(sizes);
+			return rl_fail(REND_EINVAL, "BLAS repack failed");
+		}
+		hipError_t e = hipMemcpy(s->arena + s->blas_dev[b], stage,
+		  (size_t)sizes[b], hipMemcpyHostToDevice);
+		if (e != hipSuccess) {
+			free(stage);
+			free(sizes);
+			return rl_hipfail(e, "BLAS upload");
+		}
+	}
+	free(stage);
+	free(sizes);
+	s->blas_bytes = total;
+
+	/* Normals and materials for shading. */
+	size_t nb = (size_t)s->maxtris * sizeof(struct rnrm);
+	size_t mb = (size_t)s->maxmtls * sizeof(struct rmtl);
+	hipError_t e = hipSuccess;
+	if (nb > s->nrms_cap) {
+		dfree((void **)&s->nrms);
+		if ((e = hipMalloc((void **)&s->nrms, nb)) != hipSuccess)
+			return rl_hipfail(e, "normals");
+		s->nrms_cap = nb;
+	}
+	if (mb > s->mtls_cap) {
+		dfree((void **)&s->mtls);
+		if ((e = hipMalloc((void **)&s->mtls, mb)) != hipSuccess)
+			return rl_hipfail(e, "materials");
+		s->mtls_cap = mb;
+	}
+	if ((nb && (e = hipMemcpy(s->nrms, rd->nrms, nb, hipMemcpyHostToDevice))
+	      != hipSuccess) ||
+	    (mb && (e = hipMemcpy(s->mtls, rd->mtls, mb, hipMemcpyHostToDevice))
+	      != hipSuccess))
+		return rl_hipfail(e, "shading data upload");
+	s->static_ok = 1;
+	s->dynamic_ok = 0;
+	return 0;
+}
+
+static int upload_dynamic(struct rdata *rd, struct rl_scene *s)
+{
+	if (!s->static_ok)
+		return rl_fail(REND_ESTATE, "rend_prepdynamic before rend_prepstatic");
+	const struct bnode8 *root = &rd->bnodes[rd->tlasofs];
+	size_t limit = 2 * (size_t)(s->maxinsts ? s->maxinsts : 1) *
+	  sizeof(struct bnode8);
+	long long sz = bvh8_repack_size(root, 0, limit, NULL, NULL);
+	if (sz < 0)
+		return rl_fail(REND_EINVAL, "malformed TLAS");
+	size_t at = (s->blas_bytes + 255) & ~(size_t)255;
+	int r = arena_reserve(s, at + (size_t)sz, s->blas_bytes);
+	if (r)
+		return r;
+	unsigned char *stage = aligned_alloc(64, ((size_t)sz + 63) & ~(size_t)63);
+	struct dinst *di = malloc(((size_t)rd->instcnt + 1) * sizeof(*di));
+	if (!stage || !di) {
+		free(stage);
+		free(di);
+		return rl_fail(REND_ENOMEM, "TLAS staging");
+	}
+	if (bvh8_repack(stage, (size_t)sz, root, 0, limit) != sz) {
+		free(stage);
+		free(di);
+		return rl_fail(REND_EINVAL, "TLAS repack failed");
+	}
+	for (unsigned i = 0; i < rd->instcnt; i++) {
+		const struct rinst *ri = &rd->insts[i];
+		int b = blas_find(s, ri->triofs);
+		if (b < 0) {
+			free(stage);
+			free(di);
+			return rl_fail(REND_ESTATE,
+			  "instance %u references a BLAS not built by rend_prepstatic", i);
+		}
+		memcpy(di[i].inv, ri->globinv, sizeof(di[i].inv));
+		di[i].blas = s->blas_dev[b];
+		di[i].triofs = ri->triofs;
+		di[i].flags = ri->flags;
+	}
+	hipError_t e;
+	size_t ib = ((size_t)rd->instcnt + 1) * sizeof(*di);
+	if (ib > s->insts_cap) {
+		dfree((void **)&s->insts);
+		if ((e = hipMalloc((void **)&s->insts, ib)) != hipSuccess) {
+			free(stage);
+			free(di);
+			return rl_hipfail(e, "instances");
+		}
+		s->insts_cap = ib;
+	}
+	e = hipMemcpy(s->arena + at, stage, (size_t)sz, hipMemcpyHostToDevice);
+	if (e == hipSuccess && rd->instcnt)
+		e = hipMemcpy(s->insts, di, (size_t)rd->instcnt * sizeof(*di),
+		  hipMemcpyHostToDevice);
+	free(stage);
+	free(di);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "TLAS upload");
+	s->tlas = at;
+	s->arena_used = at + (size_t)sz;
+	s->tlas_bytes = (size_t)sz;
+	s->ninsts = rd->instcnt;
+	s->dynamic_ok = 1;
+	return 0;
+}
+
+void rend_prepstatic(struct rdata *rd)
+{
+	for (unsigned j = 0; j < rd->instcnt; j++) {
+		struct rinst *ri = &rd->insts[j];
+		unsigned triofs = ri->triofs;
+		struct bnode8 *rn = &rd->bnodes[(size_t)triofs << 1];
+		if (((unsigned *)&rn->children8)[0])
+			continue;  /* already built (rend.c:3039) */
+		unsigned tricnt = ri->tricnt;
+		const struct rtri *tp = &rd->tris[triofs];
+		unsigned *imap = malloc(((size_t)tricnt + 1) * sizeof(*imap));
+		struct aabb *ab = malloc(((size_t)tricnt + 1) * sizeof(*ab));
+		if (!imap || !ab) {
+			free(imap);
+			free(ab);
+			rl_fail(REND_ENOMEM, "BLAS build buffers");
+			return;
+		}
+		struct vec3 rmin = {FLT_MAX, FLT_MAX, FLT_MAX};
+		struct vec3 rmax = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+		for (unsigned i = 0; i < tricnt; i++) {
+			ab[i].min = ab[i].max = tp[i].v0;
+			ab[i].min = rv_min(ab[i].min, tp[i].v1);
+			ab[i].max = rv_max(ab[i].max, tp[i].v1);
+			ab[i].min = rv_min(ab[i].min, tp[i].v2);
+			ab[i].max = rv_max(ab[i].max, tp[i].v2);
+			rmin = rv_min(rmin, ab[i].min);
+			rmax = rv_max(rmax, ab[i].max);
+			imap[i] = i;
+		}
+		struct bvh8_stats st;
+		bvh8_build(rn, ab, imap, tp, tricnt, rmin, rmax, &st);
+		struct rl_scene *s = rl_scene(rd);
+		if (s)
+			s->hazards |= st.hazards;
+		free(ab);
+		free(imap);
+	}
+	struct rl_scene *s = scene_add(rd);
+	if (!s) {
+		rl_fail(REND_ENOMEM, "scene registry full");
+		return;
+	}
+	/* Device upload happens at the first trace or frame after this. */
+	s->static_dirty = 1;
+	if (s->tlas_built)
+		s->dynamic_dirty = 1;
+}
+
+void rend_prepdynamic(struct rdata *rd)
+{
+	unsigned n = rd->instcnt;
+	unsigned *imap = malloc(((size_t)n + 1) * sizeof(*imap));
+	if (!imap) {
+		rl_fail(REND_ENOMEM, "TLAS build buffers");
+		return;
+	}
+	struct vec3 rmin = {FLT_MAX, FLT_MAX, FLT_MAX};
+	struct vec3 rmax = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	for (unsigned i = 0; i < n; i++) {
+		rmin = rv_min(rmin, rd->aabbs[i].min);
+		rmax = rv_max(rmax, rd->aabbs[i].max);
+		imap[i] = i;
+	}
+	struct bvh8_stats st;
+	bvh8_build(&rd->bnodes[rd->tlasofs], rd->aabbs, imap, NULL, n, rmin,
+	  rmax, &st);
+	free(imap);
+	struct rl_scene *s = scene_add(rd);
+	if (!s) {
+		rl_fail(REND_ENOMEM, "scene registry full");
+		return;
+	}
+	s->hazards |= st.hazards;
+	s->tlas_built = 1;
+	s->dynamic_dirty = 1;
+}
+
+/* Uploads whatever rend_prepstatic/rend_prepdynamic rebuilt since the last
+ * upload. */
+int rl_sync_scene(struct rl_scene *s)
+{
+	int r = 0;
+	if (s->static_dirty) {
+		if ((r = upload_static(s->rd, s)))
+			return r;
+		s->static_dirty = 0;
+		if (s->tlas_built)
+			s->dynamic_dirty = 1;
+	}
+	if (s->dynamic_dirty) {
+		if ((r = upload_dynamic(s->rd, s)))
+			return r;
+		s->dynamic_dirty = 0;
+	}
+	return 0;
+}
+
+int rend_hip_upload(struct rdata *rd)
+{
+	struct rl_scene *s = scene_add(rd);
+	if (!s)
+		return rl_fail(REND_ENOMEM, "scene registry full");
+	s->static_dirty = 1;
+	s->dynamic_dirty = s->tlas_built;
+	return rl_sync_scene(s);
+}
+
+unsigned int rend_hip_hazards(struct rdata *rd)
+{
+	struct rl_scene *s = rl_scene(rd);
+	return s ? s->hazards : 0;
+}
+
+long long rend_hip_arena_bytes(struct rdata *rd)
+{
+	struct rl_scene *s = rl_scene(rd);
+	return s && s->dynamic_ok ? (long long)s->arena_used : -1;
+}
+
+/* ---- accumulation ------------------------------------------------------- */
+
+void rend_resaccum(struct rdata *rd, unsigned int w, unsigned int h)
+{
+	free(rd->acc);
+	rd->acc = zalloc64((size_t)w * h * sizeof(*rd->acc));
+	rd->width = w;
+	rd->height = h;
+	rend_clraccum(rd);
+}
+
+void rend_clraccum(struct rdata *rd)
+{
+	if (rd->acc)
+		memset(rd->acc, 0, (size_t)rd->width * rd->height * sizeof(*rd->acc));
+	rd->samples = 0;
+	struct rl_scene *s = rl_scene(rd);
+	if (s)
+		s->acc_dirty = 1;
+}
+
+/* ---- ray-buffer entry points ---------------------------------------------- */
+
+int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
+                   void *d_occ, unsigned n, int closest, int count,
+                   hipStream_t st)
+{
+	unsigned grid;
+	int r = rl_trace_grid(closest, count, &grid);
+	if (r)
+		return r;
+	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
+	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
+	  d_occ, n, G.ctl + 2 * slot, G.spill, count ? G.stats : NULL,
+	  closest, count, G.rcp_ieee, grid, G.refill, G.max_iter, st);
+	if (e)
+		return rl_hipfail(e, "trace launch");
+	G.last_slot = slot;
+	return 0;
+}
+
+static int check_ready(struct rdata *rd, struct rl_scene **out)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->tlas_built)
+		return rl_fail(REND_ESTATE, "scene not prepared (call rend_prepstatic "
+		  "and rend_prepdynamic first)");
+	if ((r = rl_sync_scene(s)))
+		return r;
+	*out = s;
+	return 0;
+}
+
+/* Reads and clears the error word of the last launch (synchronises). */
+int rl_check_errors(hipStream_t st)
+{
+	hipError_t e = hipStreamSynchronize(st);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "trace");
+	unsigned w[2];
+	e = hipMemcpy(w, G.ctl + 2 * G.last_slot, sizeof(w), hipMemcpyDeviceToHost);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "trace status");
+	if (w[1]) {
+		hipMemset(G.ctl + 2 * G.last_slot + 1, 0, sizeof(unsigned));
+		return rl_fail(REND_EOVERFLOW, "traversal %s",
+		  (w[1] & RK_ERR_STACK) ? "stack overflow (64 entries)" :
+		  "iteration limit");
+	}
+	return 0;
+}
+
+int rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
+                   struct rhit *d_hits, unsigned int n, void *stream,
+                   struct rtrace_stats *stats)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	if (stats) {
+		hipError_t e = hipMemsetAsync(G.stats, 0,
+		  4 * sizeof(unsigned long long), st);
+		if (e != hipSuccess)
+			return rl_hipfail(e, "stats clear");
+	}
+	r = rl_trace_async(s, d_rays, d_hits, NULL, n, 1, stats != NULL, st);
+	if (r || !stats)
+		return r;
+	unsigned long long c[4];
+	hipError_t e = hipMemcpyAsync(c, G.stats, sizeof(c), hipMemcpyDeviceToHost,
+	  st);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(st);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "stats read");
+	stats->nodes += c[0];
+	stats->leaves += c[1];
+	stats->insts += c[2];
+	stats->rays += c[3];
+	return rl_check_errors(st);
+}
+
+int rend_hip_occluded(struct rdata *rd, const struct rray *d_rays,
+                      unsigned char *d_occ, unsigned int n, void *stream)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	return rl_trace_async(s, d_rays, NULL, d_occ, n, 0, 0, st);
+}
+
+static int host_roundtrip(struct rdata *rd, const struct rray *rays,
+                          void *out, size_t outsz, unsigned n, int closest,
+                          struct rtrace_stats *stats)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	if (n == 0)
+		return 0;
+	void *dr = NULL, *dout = NULL;
+	hipError_t e = hipMalloc(&dr, (size_t)n * sizeof(*rays));
+	if (e == hipSuccess)
+		e = hipMalloc(&dout, (size_t)n * outsz);
+	if (e != hipSuccess) {
+		dfree(&dr);
+		return rl_hipfail(e, "ray buffers");
+	}
+	e = hipMemcpy(dr, rays, (size_t)n * sizeof(*rays), hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		r = rl_hipfail(e, "ray upload");
+		goto out;
+	}
+	if (closest)
+		r = rend_hip_trace(rd, dr, dout, n, G.stream, stats);
+	else
+		r = rend_hip_occluded(rd, dr, dout, n, G.stream);
+	if (!r)
+		r = rl_check_errors(G.stream);
+	if (!r) {
+		e = hipMemcpy(out, dout, (size_t)n * outsz, hipMemcpyDeviceToHost);
+		if (e != hipSuccess)
+			r = rl_hipfail(e, "result download");
+	}
+out:
+	dfree(&dr);
+	dfree(&dout);
+	return r;
+}
+
+int rend_hip_trace_host(struct rdata *rd, const struct rray *rays,
+                        struct rhit *hits, unsigned int n,
+                        struct rtrace_stats *stats)
+{
+	return host_roundtrip(rd, rays, hits, sizeof(*hits), n, 1, stats);
+}
+
+int rend_hip_occluded_host(struct rdata *rd, const struct rray *rays,
+                           unsigned char *occ, unsigned int n)
+{
+	return host_roundtrip(rd, rays, occ, 1, n, 0, NULL);
+}
+
+/* ---- device memory and timing helpers --------------------------------------- */
+
+void *rend_hip_malloc(unsigned long long bytes)
+{
+	if (rl_device())
+		return NULL;
+	void *p = NULL;
+	hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+	if (e != hipSuccess) {
+		rl_hipfail(e, "rend_hip_malloc");
+		return NULL;
+	}
+	return p;
+}
+
+int rend_hip_free(void *d)
+{
+	hipError_t e = hipFree(d);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "rend_hip_free");
+}
+
+int rend_hip_memcpy_htod(void *d, const void *h, unsigned long long bytes)
+{
+	hipError_t e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "htod");
+}
+
+int rend_hip_memcpy_dtoh(void *h, const void *d, unsigned long long bytes)
+{
+	hipError_t e = hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "dtoh");
+}
+
+int rend_hip_sync(void)
+{
+	if (!G.ready)
+		return 0;
+	hipError_t e = hipStreamSynchronize(G.stream);
+	if (e == hipSuccess)
+		e = hipDeviceSynchronize();
+	return e == hipSuccess ? 0 : rl_hipfail(e, "sync");
+}
+
+void *rend_hip_stream(void)
+{
+	return rl_device() ? NULL : (void *)G.stream;
+}
+
+void *rend_hip_event_create(void)
+{
+	if (rl_device())
+		return NULL;
+	hipEvent_t ev;
+	return hipEventCreate(&ev) == hipSuccess ? (void *)ev : NULL;
+}
+
+int rend_hip_event_record(void *ev, void *stream)
+{
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	hipError_t e = hipEventRecord((hipEvent_t)ev, st);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "event record");
+}
+
+float rend_hip_event_elapsed(void *start, void *stop)
+{
+	float ms = -1.0f;
+	if (hipEventSynchronize((hipEvent_t)stop) != hipSuccess)
+		return -1.0f;
+	if (hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)stop)
+	    != hipSuccess)
+		return -1.0f;
+	return ms;
+}
+
+int rend_hip_event_destroy(void *ev)
+{
+	return hipEventDestroy((hipEvent_t)ev) == hipSuccess ? 0 : REND_ENODEV;
+}

@@ -1,0 +1,181 @@
+// Frame kernels behind rend_render: camera ray generation, the reference's
+// primary-ray shading and progressive accumulation.
+//
+//   k_camrays   make_camray_pckt (rend.c:3293-3402): jittered pinhole or
+//               thin-lens rays; directions normalised exactly (the reference
+//               uses the approximate RSQRTPS) and jitter from a counter-based
+//               hash instead of xorshift128+ packets, so images agree with the
+//               reference statistically, not bit for bit.
+//   k_shade_normals  trace_rays/calc_nrm (rend.c:3404-3440, 3109-3116):
+//               colour (n * 0.5 + 0.5) * albedo, background on a miss.
+//   accumulate  accum_pckt (rend.c:3565-3580): acc += colour, ARGB8 from
+//               acc / (spp + samples) with the reference's 8-bit wrap.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../dlayout.h"
+#include "rend_rays.h"
+
+#include "rng.h"
+
+struct RNrm {  // rend.h:73-80
+  float n0[3];
+  uint32_t mtlid;
+  float n1[3];
+  uint32_t pad0;
+  float n2[3];
+  uint32_t pad1;
+};
+
+struct RMtl {  // rend.h:54-62
+  float col[3];
+  float metallic, roughness, ior;
+  uint32_t flags, pad0;
+};
+
+namespace {
+
+__device__ __forceinline__ void camray(const CamArgs &c, float px, float py,
+                                       float u0, float u1, float u2, float u3,
+                                       rray &r) {
+  float sx = px + (u0 - 0.5f);
+  float sy = py + (u1 - 0.5f);
+  sx = __builtin_fmaf(sx, c.rw, -0.5f);
+  sy = __builtin_fmaf(sy, c.rh, -0.5f);
+  sx = sx * c.fovfdist;
+  sy = sy * c.fovfdist;
+  sx = sx * c.aspect;
+  float ox = c.eye[0], oy = c.eye[1], oz = c.eye[2];
+  float tx, ty, tz;
+  const float rx0 = __builtin_fmaf(c.ri[0], sx, -(c.up[0] * sy));
+  const float ry0 = __builtin_fmaf(c.ri[1], sx, -(c.up[1] * sy));
+  const float rz0 = __builtin_fmaf(c.ri[2], sx, -(c.up[2] * sy));
+  if (c.dof) {
+    const float rad = sqrtf(u2);
+    const float th = 6.28318531f * u3;
+    float s, co;
+    sincosf(th, &s, &co);
+    const float lx = co * rad, ly = s * rad;
+    ox = __builtin_fmaf(__builtin_fmaf(c.ri[0], lx, c.up[0] * ly), c.focrad, c.eye[0]);
+    oy = __builtin_fmaf(__builtin_fmaf(c.ri[1], lx, c.up[1] * ly), c.focrad, c.eye[1]);
+    oz = __builtin_fmaf(__builtin_fmaf(c.ri[2], lx, c.up[2] * ly), c.focrad, c.eye[2]);
+    tx = (__builtin_fmaf(c.fwd[0], c.fdist, c.eye[0]) + rx0) - ox;
+    ty = (__builtin_fmaf(c.fwd[1], c.fdist, c.eye[1]) + ry0) - oy;
+    tz = (__builtin_fmaf(c.fwd[2], c.fdist, c.eye[2]) + rz0) - oz;
+  } else {
+    tx = __builtin_fmaf(c.fwd[0], c.fdist, rx0);
+    ty = __builtin_fmaf(c.fwd[1], c.fdist, ry0);
+    tz = __builtin_fmaf(c.fwd[2], c.fdist, rz0);
+  }
+  const float il = 1.0f / sqrtf(__builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz)));
+  r.ox = ox; r.oy = oy; r.oz = oz; r.tmax = 3.402823466e+38f;
+  r.dx = tx * il; r.dy = ty * il; r.dz = tz * il; r.pad = 0.0f;
+}
+
+// Rays for pixels of a rw x rh region, spp consecutive rays per pixel.
+__global__ void k_camrays(CamArgs c, uint32_t rw, uint32_t rh, uint32_t spp,
+                          uint32_t seed, uint32_t frame, rray *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = rw * rh * spp;
+  if (i >= n) return;
+  const uint32_t p = i / spp;
+  const uint32_t x = p % rw, y = p / rw;
+  const uint32_t key = rng_key(seed, frame, i);
+  rray r;
+  camray(c, (float)x, (float)y, rng_u01(key, 0), rng_u01(key, 1),
+         rng_u01(key, 2), rng_u01(key, 3), r);
+  float4 *o = (float4 *)&out[i];
+  o[0] = make_float4(r.ox, r.oy, r.oz, r.tmax);
+  o[1] = make_float4(r.dx, r.dy, r.dz, r.pad);
+}
+
+__device__ __forceinline__ uint32_t pack8(float a) {
+  return (uint32_t)(long long)(255.0f * a) & 0xffu;
+}
+
+// Shade spp hits per pixel with the reference's normal tint and accumulate.
+__global__ void k_shade_normals(const rhit *__restrict__ hits,
+                                const dinst *__restrict__ insts,
+                                const RNrm *__restrict__ nrms,
+                                const RMtl *__restrict__ mtls, float bgr,
+                                float bgg, float bgb, uint32_t rw, uint32_t rh,
+                                uint32_t width, uint32_t spp, float invsamples,
+                                float *__restrict__ acc, uint32_t *__restrict__ buf) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= rw * rh) return;
+  float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+  for (uint32_t s = 0; s < spp; s++) {
+    const rhit h = hits[(size_t)p * spp + s];
+    float r = bgr, g = bgg, b = bgb;
+    if (h.t < 3.402823466e+38f) {
+      const uint32_t inst = h.id & 0xfffu, tri = h.id >> 12;
+      const dinst &in = insts[inst];
+      const RNrm &rn = nrms[in.triofs + tri];
+      const float u = h.u, v = h.v, w = 1.0f - u - v;
+      float nx = rn.n1[0] * u + (rn.n2[0] * v + rn.n0[0] * w);
+      float ny = rn.n1[1] * u + (rn.n2[1] * v + rn.n0[1] * w);
+      float nz = rn.n1[2] * u + (rn.n2[2] * v + rn.n0[2] * w);
+      // inverse transpose of the 3x3 part of globinv (rend.c:3431-3436)
+      const float *m = in.inv;
+      const float tx = m[0] * nx + m[4] * ny + m[8] * nz;
+      const float ty = m[1] * nx + m[5] * ny + m[9] * nz;
+      const float tz = m[2] * nx + m[6] * ny + m[10] * nz;
+      const float il = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
+      nx = tx * il; ny = ty * il; nz = tz * il;
+      const RMtl &mt = mtls[rn.mtlid];
+      r = (nx + 1.0f) * 0.5f * mt.col[0];
+      g = (ny + 1.0f) * 0.5f * mt.col[1];
+      b = (nz + 1.0f) * 0.5f * mt.col[2];
+    }
+    cr += r; cg += g; cb += b;
+  }
+  const uint32_t x = p % rw, y = p / rw;
+  const size_t o = (size_t)y * width + x;
+  const float ar = acc[3 * o + 0] + cr;
+  const float ag = acc[3 * o + 1] + cg;
+  const float ab = acc[3 * o + 2] + cb;
+  acc[3 * o + 0] = ar;
+  acc[3 * o + 1] = ag;
+  acc[3 * o + 2] = ab;
+  buf[o] = 0xff000000u | pack8(ar * invsamples) << 16 |
+           pack8(ag * invsamples) << 8 | pack8(ab * invsamples);
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+extern "C" int rk_camrays(const CamArgs *c, unsigned rw, unsigned rh,
+                          unsigned spp, unsigned seed, unsigned frame,
+                          void *out, void *stream) {
+  const uint32_t n = rw * rh * spp;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_camrays, dim3((n + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, *c, rw, rh, spp, seed, frame, (rray *)out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_shade_normals(const void *hits, const void *insts,
+                                const void *nrms, const void *mtls,
+                                const float bg[3], unsigned rw, unsigned rh,
+                                unsigned width, unsigned spp, float invsamples,
+                                float *acc, unsigned *buf, void *stream) {
+  const uint32_t n = rw * rh;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_shade_normals, dim3((n + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, (const rhit *)hits, (const dinst *)insts,
+                     (const RNrm *)nrms, (const RMtl *)mtls, bg[0], bg[1], bg[2],
+                     rw, rh, width, spp, invsamples, acc, buf);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_fill_f32(float *p, unsigned long long n, float v, void *stream) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, p, (size_t)n, v);
+  return (int)hipGetLastError();
+}

@@ -1,0 +1,461 @@
+// Closest-hit and any-hit traversal of the two-level BVH8 on CDNA4 (gfx950).
+//
+// Semantics follow the reference's single-ray traversal operation by
+// operation: intersect_tlas (rend.c:1976-2155), the instance transform
+// mulpos/muldir (rend.c:171-187), intersect_blas (rend.c:714-968) and the
+// any-hit pair intersect_any_tlas/blas (rend.c:2883-2998, 1797-1973):
+//   * slab test t = fma(plane, 1/d, -o/d) with x86 MAXPS/MINPS rules (the
+//     second operand wins on NaN) and an ordered <= test;
+//   * children pushed in the node's per-octant rank order, nearest visited
+//     first; stack entries whose entry distance exceeds the current t are
+//     skipped when popped (the reference compacts them eagerly, which removes
+//     the same entries);
+//   * Moeller-Trumbore with the reference's FMA shapes and the host CPU's
+//     RCPSS reciprocal reproduced from a 2048-entry table;
+//   * among hits in a leaf, the smallest t and, on ties, the highest lane.
+// The file is compiled with -ffp-contract=off; fused operations are explicit.
+//
+// Execution model: one ray per lane, wave64, persistent workgroups.  Each
+// wave keeps 64 lanes busy by refilling idle lanes from a global ray counter
+// with one atomic per refill (ballot + mbcnt prefix gives each idle lane its
+// ray).  The stack's first REND_LDS_STACK entries live in LDS, deeper entries
+// in a per-thread global spill area.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../dlayout.h"
+#include "../rcp_table.h"
+#include "rend_rays.h"
+
+#define FLT_MAXF 3.402823466e+38f
+
+__constant__ uint32_t c_rcp_tab[2048] = RCP_TABLE_INIT;
+
+namespace {
+
+struct RayS {
+  float ox, oy, oz, dx, dy, dz;
+  float idx, idy, idz, rx, ry, rz;
+  uint32_t oct;
+};
+
+__device__ __forceinline__ float xmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float xmin(float a, float b) { return a < b ? a : b; }
+
+__device__ __forceinline__ void ray_setup(RayS &r, float ox, float oy, float oz,
+                                          float dx, float dy, float dz) {
+  r.ox = ox; r.oy = oy; r.oz = oz;
+  r.dx = dx; r.dy = dy; r.dz = dz;
+  r.idx = 1.0f / dx;
+  r.idy = 1.0f / dy;
+  r.idz = 1.0f / dz;
+  r.rx = ox * r.idx;
+  r.ry = oy * r.idy;
+  r.rz = oz * r.idz;
+  r.oct = (dz >= 0.0f ? 4u : 0u) | (dy >= 0.0f ? 2u : 0u) | (dx >= 0.0f ? 1u : 0u);
+}
+
+// x86 RCPSS from the captured mantissa table (rule proven exhaustively by
+// gen_rcp_table.c).
+__device__ __forceinline__ float rcp_table(float x, const uint32_t *tab) {
+  uint32_t u = __float_as_uint(x);
+  uint32_t s = u & 0x80000000u;
+  uint32_t e = (u >> 23) & 0xffu;
+  uint32_t m = u & 0x7fffffu;
+  uint32_t r = s | ((253u - e) << 23) | tab[m >> 12];
+  r = (e >= 253u) ? s : r;
+  r = (e == 255u && m != 0u) ? (u | 0x00400000u) : r;
+  r = (e == 0u) ? (s | 0x7f800000u) : r;
+  return __uint_as_float(r);
+}
+
+struct Stack {
+  uint2 (*lds)[REND_TRACE_BLOCK];
+  uint2 *spill;
+  uint32_t tid, gtid, nthreads;
+  __device__ __forceinline__ void put(uint32_t i, uint32_t ofs, float d) {
+    uint2 e = make_uint2(ofs, __float_as_uint(d));
+    if (i < REND_LDS_STACK) lds[i][tid] = e;
+    else spill[(size_t)(i - REND_LDS_STACK) * nthreads + gtid] = e;
+  }
+  __device__ __forceinline__ uint2 get(uint32_t i) const {
+    if (i < REND_LDS_STACK) return lds[i][tid];
+    return spill[(size_t)(i - REND_LDS_STACK) * nthreads + gtid];
+  }
+};
+
+template <bool CLOSEST, bool COUNT, bool RCP_IEEE>
+__global__ __launch_bounds__(REND_TRACE_BLOCK)
+void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
+             const dinst *__restrict__ insts, const rray *__restrict__ rays,
+             rhit *__restrict__ hits, uint8_t *__restrict__ occ, uint32_t n,
+             uint32_t *__restrict__ ctl, uint2 *__restrict__ spill,
+             unsigned long long *__restrict__ stats, uint32_t refill,
+             uint32_t max_iter) {
+  __shared__ uint32_t s_tab[2048];
+  __shared__ uint2 s_stack[REND_LDS_STACK][REND_TRACE_BLOCK];
+  if (!RCP_IEEE) {
+    for (uint32_t i = threadIdx.x; i < 2048; i += REND_TRACE_BLOCK)
+      s_tab[i] = c_rcp_tab[i];
+    __syncthreads();
+  }
+
+  const uint32_t lane = threadIdx.x & 63u;
+  Stack st;
+  st.lds = s_stack;
+  st.spill = spill;
+  st.tid = threadIdx.x;
+  st.gtid = blockIdx.x * REND_TRACE_BLOCK + threadIdx.x;
+  st.nthreads = gridDim.x * REND_TRACE_BLOCK;
+
+  bool active = false;
+  bool exhausted = false;  // wave-uniform
+  uint32_t rid = 0;
+  RayS wr, cr;
+  float t = 0.0f, tu = 0.0f, tv = 0.0f;
+  uint32_t hid = 0xffffffffu;
+  uint32_t ofs = 0, sp = 0, blas_sp = 0, instid = 0, iter = 0;
+  uint64_t base = tlas;
+  bool in_blas = false;
+  bool occluded = false;
+  uint64_t c_nodes = 0, c_leaves = 0, c_insts = 0, c_rays = 0;
+
+  for (;;) {
+    // ---- refill idle lanes from the global ray counter ------------------
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = (uint32_t)__popcll(idle);
+    if (!exhausted && nidle >= refill) {
+      uint32_t first = 0;
+      if (lane == 0) first = atomicAdd(&ctl[0], nidle);
+      first = __shfl(first, 0);
+      if (first + nidle >= n) exhausted = true;
+      if (!active) {
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+            (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        const uint32_t r = first + below;
+        if (r < n) {
+          rid = r;
+          const float4 a = *(const float4 *)&rays[r].ox;
+          const float4 b = *(const float4 *)&rays[r].dx;
+          ray_setup(wr, a.x, a.y, a.z, b.x, b.y, b.z);
+          cr = wr;
+          t = a.w;
+          tu = tv = 0.0f;
+          hid = 0xffffffffu;
+          ofs = 0; sp = 0; blas_sp = 0; iter = 0;
+          base = tlas;
+          in_blas = false;
+          occluded = false;
+          active = true;
+          if (COUNT) c_rays++;
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (exhausted) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one traversal step -------------------------------------------------
+    bool need_pop = false;
+    bool done = false;
+    if (++iter > max_iter) {
+      atomicOr(&ctl[1], RK_ERR_ITER);
+      done = true;
+    } else if (!(ofs & DNODE_LEAF)) {
+      const dnode *nd = (const dnode *)(arena + base + ofs);
+      const float *nx = (cr.oct & 1u) ? nd->minx : nd->maxx;
+      const float *fx = (cr.oct & 1u) ? nd->maxx : nd->minx;
+      const float *ny = (cr.oct & 2u) ? nd->miny : nd->maxy;
+      const float *fy = (cr.oct & 2u) ? nd->maxy : nd->miny;
+      const float *nz = (cr.oct & 4u) ? nd->minz : nd->maxz;
+      const float *fz = (cr.oct & 4u) ? nd->maxz : nd->minz;
+      float nxv[8], fxv[8], nyv[8], fyv[8], nzv[8], fzv[8];
+      uint32_t ch[8];
+#pragma unroll
+      for (int k = 0; k < 8; k += 4) {
+        *(float4 *)&nxv[k] = *(const float4 *)&nx[k];
+        *(float4 *)&fxv[k] = *(const float4 *)&fx[k];
+        *(float4 *)&nyv[k] = *(const float4 *)&ny[k];
+        *(float4 *)&fyv[k] = *(const float4 *)&fy[k];
+        *(float4 *)&nzv[k] = *(const float4 *)&nz[k];
+        *(float4 *)&fzv[k] = *(const float4 *)&fz[k];
+        *(uint4 *)&ch[k] = *(const uint4 *)&nd->child[k];
+      }
+      const uint32_t invw = nd->inv[cr.oct];
+      if (COUNT) c_nodes++;
+
+      float tmn[8];
+      uint32_t hmr = 0;
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const float t0x = __builtin_fmaf(nxv[c], cr.idx, -cr.rx);
+        const float t0y = __builtin_fmaf(nyv[c], cr.idy, -cr.ry);
+        const float t0z = __builtin_fmaf(nzv[c], cr.idz, -cr.rz);
+        const float t1x = __builtin_fmaf(fxv[c], cr.idx, -cr.rx);
+        const float t1y = __builtin_fmaf(fyv[c], cr.idy, -cr.ry);
+        const float t1z = __builtin_fmaf(fzv[c], cr.idz, -cr.rz);
+        const float lo = xmax(xmax(xmax(t0x, t0y), t0z), 0.0f);
+        const float hi = xmin(xmin(xmin(t1x, t1y), t1z), t);
+        tmn[c] = lo;
+        const uint32_t rank = (invw >> (3 * c)) & 7u;
+        hmr |= (lo <= hi ? 1u : 0u) << rank;
+      }
+
+      const uint32_t hitcnt = __builtin_popcount(hmr);
+      if (hitcnt == 0) {
+        need_pop = true;
+      } else {
+        // Highest rank among hits: nearest (closest hit) or, for the any-hit
+        // pass, the reference's unordered push makes the highest child index
+        // the next node; ranks then order by child index.
+        uint32_t next = 0;
+        if (CLOSEST) {
+          const uint32_t top = 31u - __builtin_clz(hmr);
+#pragma unroll
+          for (int c = 0; c < 8; c++) {
+            const uint32_t rank = (invw >> (3 * c)) & 7u;
+            if ((hmr >> rank) & 1u) {
+              if (rank == top) {
+                next = ch[c];
+              } else {
+                const uint32_t pos = __builtin_popcount(hmr & ((1u << rank) - 1u));
+                st.put(sp + pos, ch[c], tmn[c]);
+              }
+            }
+          }
+        } else {
+          // Unordered: child index order (rend.c:1884-1886).
+          uint32_t hm = 0;
+#pragma unroll
+          for (int c = 0; c < 8; c++) {
+            const uint32_t rank = (invw >> (3 * c)) & 7u;
+            hm |= ((hmr >> rank) & 1u) << c;
+          }
+          const uint32_t top = 31u - __builtin_clz(hm);
+#pragma unroll
+          for (int c = 0; c < 8; c++) {
+            if ((hm >> c) & 1u) {
+              if ((uint32_t)c == top) {
+                next = ch[c];
+              } else {
+                const uint32_t pos = __builtin_popcount(hm & ((1u << c) - 1u));
+                st.put(sp + pos, ch[c], 0.0f);
+              }
+            }
+          }
+        }
+        if (sp + hitcnt > REND_STACK_MAX) {
+          atomicOr(&ctl[1], RK_ERR_STACK);
+          done = true;
+        } else {
+          sp += hitcnt - 1;
+          ofs = next;
+        }
+      }
+    } else if (in_blas) {
+      const dleaf *lf = (const dleaf *)(arena + base + (ofs & ~DNODE_LEAF));
+      float4 v0x = *(const float4 *)lf->v0x, v0y = *(const float4 *)lf->v0y,
+             v0z = *(const float4 *)lf->v0z;
+      float4 e0x = *(const float4 *)lf->e0x, e0y = *(const float4 *)lf->e0y,
+             e0z = *(const float4 *)lf->e0z;
+      float4 e1x = *(const float4 *)lf->e1x, e1y = *(const float4 *)lf->e1y,
+             e1z = *(const float4 *)lf->e1z;
+      const float *V0x = (const float *)&v0x, *V0y = (const float *)&v0y,
+                  *V0z = (const float *)&v0z;
+      const float *E0x = (const float *)&e0x, *E0y = (const float *)&e0y,
+                  *E0z = (const float *)&e0z;
+      const float *E1x = (const float *)&e1x, *E1y = (const float *)&e1y,
+                  *E1z = (const float *)&e1z;
+      if (COUNT) c_leaves++;
+
+      float tt[4], uu[4], vv[4];
+      uint32_t mask = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float pvx = __builtin_fmaf(cr.dy, E1z[k], -(cr.dz * E1y[k]));
+        const float pvy = __builtin_fmaf(cr.dz, E1x[k], -(cr.dx * E1z[k]));
+        const float pvz = __builtin_fmaf(cr.dx, E1y[k], -(cr.dy * E1x[k]));
+        const float tvx = cr.ox - V0x[k];
+        const float tvy = cr.oy - V0y[k];
+        const float tvz = cr.oz - V0z[k];
+        const float det = __builtin_fmaf(E0x[k], pvx,
+                          __builtin_fmaf(E0y[k], pvy, E0z[k] * pvz));
+        const float qvx = __builtin_fmaf(tvy, E0z[k], -(tvz * E0y[k]));
+        const float qvy = __builtin_fmaf(tvz, E0x[k], -(tvx * E0z[k]));
+        const float qvz = __builtin_fmaf(tvx, E0y[k], -(tvy * E0x[k]));
+        const float idet = RCP_IEEE ? 1.0f / det : rcp_table(det, s_tab);
+        const float u = idet * __builtin_fmaf(tvx, pvx, __builtin_fmaf(tvy, pvy, tvz * pvz));
+        const float v = idet * __builtin_fmaf(cr.dx, qvx, __builtin_fmaf(cr.dy, qvy, cr.dz * qvz));
+        const float tk = idet * __builtin_fmaf(E1x[k], qvx, __builtin_fmaf(E1y[k], qvy, E1z[k] * qvz));
+        const bool ok = (u >= 0.0f) & (v >= 0.0f) & ((u + v) <= 1.0f) &
+                        (tk > 0.0f) & (tk < t);
+        tt[k] = tk; uu[k] = u; vv[k] = v;
+        mask |= (ok ? 1u : 0u) << k;
+      }
+      if (!CLOSEST) {
+        if (mask) { done = true; occluded = true; }
+        else need_pop = true;
+      } else {
+        if (mask) {
+          float tb[4];
+#pragma unroll
+          for (int k = 0; k < 4; k++) tb[k] = ((mask >> k) & 1u) ? tt[k] : FLT_MAXF;
+          const float m = xmin(xmin(tb[0], tb[1]), xmin(tb[2], tb[3]));
+          int sel = 0;
+#pragma unroll
+          for (int k = 0; k < 4; k++) if (tb[k] == m) sel = k;
+          float su = uu[0], sv = vv[0], stt = tb[0];
+#pragma unroll
+          for (int k = 1; k < 4; k++)
+            if (sel == k) { su = uu[k]; sv = vv[k]; stt = tb[k]; }
+          t = stt;
+          tu = su;
+          tv = sv;
+          hid = (lf->id[sel] << 12) | instid;
+        }
+        need_pop = true;
+      }
+    } else {
+      // TLAS leaf: enter the instance's BLAS with the transformed ray.
+      instid = ofs & ~DNODE_LEAF;
+      const dinst *in = insts + instid;
+      const float4 m0 = *(const float4 *)&in->inv[0];
+      const float4 m1 = *(const float4 *)&in->inv[4];
+      const float4 m2 = *(const float4 *)&in->inv[8];
+      const uint64_t blas = in->blas;
+      if (COUNT) c_insts++;
+      const float ox = m0.x * wr.ox + m0.y * wr.oy + m0.z * wr.oz + m0.w;
+      const float oy = m1.x * wr.ox + m1.y * wr.oy + m1.z * wr.oz + m1.w;
+      const float oz = m2.x * wr.ox + m2.y * wr.oy + m2.z * wr.oz + m2.w;
+      const float dx = m0.x * wr.dx + m0.y * wr.dy + m0.z * wr.dz;
+      const float dy = m1.x * wr.dx + m1.y * wr.dy + m1.z * wr.dz;
+      const float dz = m2.x * wr.dx + m2.y * wr.dy + m2.z * wr.dz;
+      ray_setup(cr, ox, oy, oz, dx, dy, dz);
+      in_blas = true;
+      blas_sp = sp;
+      base = blas;
+      ofs = 0;
+    }
+
+    if (need_pop && !done) {
+      for (;;) {
+        if (in_blas && sp == blas_sp) {
+          in_blas = false;
+          cr = wr;
+          base = tlas;
+        }
+        if (sp == 0) { done = true; break; }
+        --sp;
+        const uint2 e = st.get(sp);
+        if (!CLOSEST || __uint_as_float(e.y) <= t) { ofs = e.x; break; }
+      }
+    }
+
+    if (done) {
+      if (CLOSEST) {
+        rhit h; h.t = t; h.u = tu; h.v = tv; h.id = hid;
+        *(float4 *)&hits[rid] = *(const float4 *)&h;
+      } else {
+        occ[rid] = occluded ? 1 : 0;
+      }
+      active = false;
+    }
+  }
+
+  if (COUNT) {
+    for (int o = 32; o > 0; o >>= 1) {
+      c_nodes += __shfl_down(c_nodes, o);
+      c_leaves += __shfl_down(c_leaves, o);
+      c_insts += __shfl_down(c_insts, o);
+      c_rays += __shfl_down(c_rays, o);
+    }
+    if (lane == 0) {
+      atomicAdd(&stats[0], (unsigned long long)c_nodes);
+      atomicAdd(&stats[1], (unsigned long long)c_leaves);
+      atomicAdd(&stats[2], (unsigned long long)c_insts);
+      atomicAdd(&stats[3], (unsigned long long)c_rays);
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+__global__ void k_zero_u32(uint32_t *p, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+template <bool C, bool N, bool I>
+hipError_t launch_one(uint32_t grid, hipStream_t s, const uint8_t *arena,
+                      uint64_t tlas, const dinst *insts, const rray *rays,
+                      rhit *hits, uint8_t *occ, uint32_t n, uint32_t *ctl,
+                      uint2 *spill, unsigned long long *stats, uint32_t refill,
+                      uint32_t max_iter) {
+  hipLaunchKernelGGL((k_trace<C, N, I>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
+                     arena, tlas, insts, rays, hits, occ, n, ctl, spill, stats,
+                     refill, max_iter);
+  return hipGetLastError();
+}
+
+template <bool C, bool N, bool I>
+int occ_one() {
+  int b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (k_trace<C, N, I>),
+                                                    REND_TRACE_BLOCK, 0) != hipSuccess)
+    return -1;
+  return b;
+}
+}  // namespace
+
+// C-ABI between the host library and the traversal kernels.
+extern "C" int rk_trace_occupancy(int closest, int count, int rcp_ieee) {
+  const int k = (closest ? 4 : 0) | (count ? 2 : 0) | (rcp_ieee ? 1 : 0);
+  switch (k) {
+    case 0: return occ_one<false, false, false>();
+    case 1: return occ_one<false, false, true>();
+    case 2: return occ_one<false, true, false>();
+    case 3: return occ_one<false, true, true>();
+    case 4: return occ_one<true, false, false>();
+    case 5: return occ_one<true, false, true>();
+    case 6: return occ_one<true, true, false>();
+    default: return occ_one<true, true, true>();
+  }
+}
+
+// Zeroes ctl[0..nctl) with a kernel (safe inside captured graphs), then
+// launches the persistent traversal grid.  Returns a hipError_t value.
+extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
+                               const void *insts, const void *rays, void *hits,
+                               void *occ, unsigned int n, unsigned int *ctl,
+                               void *spill, unsigned long long *stats,
+                               int closest, int count, int rcp_ieee,
+                               unsigned int grid, unsigned int refill,
+                               unsigned int max_iter, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctl, 2u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return (int)e;
+  if (n == 0) return 0;
+  const uint8_t *a = (const uint8_t *)arena;
+  const dinst *in = (const dinst *)insts;
+  const rray *r = (const rray *)rays;
+  rhit *h = (rhit *)hits;
+  uint8_t *o = (uint8_t *)occ;
+  uint2 *sp = (uint2 *)spill;
+  const int k = (closest ? 4 : 0) | (count ? 2 : 0) | (rcp_ieee ? 1 : 0);
+  switch (k) {
+    case 0: e = launch_one<false, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 1: e = launch_one<false, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 2: e = launch_one<false, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 3: e = launch_one<false, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 4: e = launch_one<true, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 5: e = launch_one<true, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 6: e = launch_one<true, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    default: e = launch_one<true, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+  }
+  return (int)e;
+}

@@ -1,0 +1,134 @@
+"""ctypes bindings of the oracle (oracle/liboracle.so, the C restatement of
+the reference's traversal) and, when it has been built from the reference's
+sources, of oracle/_ref/libref.so.  Test infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from nodebleed_amd import HIT, RAY, Scene, ptr
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE = ROOT / "oracle" / "liboracle.so"
+REF = ROOT / "oracle" / "_ref" / "libref.so"
+REF_FAST = ROOT / "oracle" / "_ref" / "libref_fast.so"
+
+_or = None
+_ref = {}
+
+
+def oracle() -> C.CDLL:
+    global _or
+    if _or is None:
+        if not ORACLE.exists():
+            subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True,
+                           capture_output=True)
+        L = C.CDLL(str(ORACLE))
+        L.or_trace.restype = C.c_int
+        L.or_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int,
+                               C.c_int, C.POINTER(C.c_ulonglong), C.c_void_p]
+        L.or_occluded.restype = C.c_int
+        L.or_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
+                                  C.c_int, C.c_int]
+        L.or_bruteforce.restype = None
+        L.or_bruteforce.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int]
+        L.or_rcp.restype = C.c_float
+        L.or_rcp.argtypes = [C.c_float]
+        for f in ("or_xmax", "or_xmin"):
+            getattr(L, f).restype = C.c_float
+            getattr(L, f).argtypes = [C.c_float, C.c_float]
+        _or = L
+    return _or
+
+
+def or_trace(scene: Scene, rays: np.ndarray, rcp_ieee=False, threads=8,
+             per_ray=False):
+    rays = np.ascontiguousarray(rays, dtype=RAY)
+    hits = np.zeros(len(rays), HIT)
+    cnt = (C.c_ulonglong * 3)()
+    pr = np.zeros((len(rays), 3), np.uint64) if per_ray else None
+    err = oracle().or_trace(C.c_void_p(scene.rd), ptr(rays), ptr(hits), len(rays),
+                            int(rcp_ieee), threads, cnt,
+                            ptr(pr) if per_ray else None)
+    assert err == 0, f"oracle stack overflow on {err} rays"
+    out = (hits, tuple(cnt))
+    return out + (pr,) if per_ray else out
+
+
+def or_occluded(scene: Scene, rays: np.ndarray, rcp_ieee=False, threads=8):
+    rays = np.ascontiguousarray(rays, dtype=RAY)
+    occ = np.zeros(len(rays), np.uint8)
+    err = oracle().or_occluded(C.c_void_p(scene.rd), ptr(rays), ptr(occ), len(rays),
+                               int(rcp_ieee), threads)
+    assert err == 0
+    return occ
+
+
+def or_bruteforce(scene: Scene, rays: np.ndarray, rcp_ieee=False):
+    rays = np.ascontiguousarray(rays, dtype=RAY)
+    hits = np.zeros(len(rays), HIT)
+    oracle().or_bruteforce(C.c_void_p(scene.rd), ptr(rays), ptr(hits), len(rays),
+                           int(rcp_ieee))
+    return hits
+
+
+def reference(fast: bool = False) -> C.CDLL | None:
+    """The reference's own code (built here from /root/reference), or None."""
+    path = REF_FAST if fast else REF
+    if path in _ref:
+        return _ref[path]
+    if not path.exists():
+        _ref[path] = None
+        return None
+    L = C.CDLL(str(path))
+    L.ref_scene.restype = C.c_void_p
+    L.ref_scene.argtypes = [C.c_void_p, C.c_uint, C.c_void_p, C.c_uint, C.c_void_p]
+    L.ref_scene_free.argtypes = [C.c_void_p]
+    L.ref_bnodes.restype = C.c_void_p
+    L.ref_bnodes.argtypes = [C.c_void_p]
+    L.ref_tlasofs.restype = C.c_uint
+    L.ref_tlasofs.argtypes = [C.c_void_p]
+    L.ref_aabbs.restype = C.c_void_p
+    L.ref_aabbs.argtypes = [C.c_void_p]
+    L.ref_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int]
+    L.ref_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int]
+    L.ref_rcpps.restype = C.c_float
+    L.ref_rcpps.argtypes = [C.c_float]
+    _ref[path] = L
+    return L
+
+
+class RefScene:
+    """The reference's rend_prepstatic/rend_prepdynamic over a Scene's data."""
+
+    def __init__(self, scene: Scene, fast: bool = False):
+        self.L = reference(fast)
+        self.tris = scene.tris().copy()
+        self.insts = scene.insts().copy()
+        self.globs = np.ascontiguousarray(scene.globs())
+        self.rd = self.L.ref_scene(ptr(self.tris), len(self.tris), ptr(self.insts),
+                                   len(self.insts), ptr(self.globs))
+
+    def arena(self, byte_ofs: int, nbytes: int) -> np.ndarray:
+        p = self.L.ref_bnodes(self.rd) + byte_ofs
+        return np.frombuffer((C.c_char * nbytes).from_address(p), np.uint8).copy()
+
+    def trace(self, rays, threads=8):
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), HIT)
+        self.L.ref_trace(C.c_void_p(self.rd), ptr(rays), ptr(hits), len(rays), threads)
+        return hits
+
+    def occluded(self, rays, threads=8):
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        occ = np.zeros(len(rays), np.uint8)
+        self.L.ref_occluded(C.c_void_p(self.rd), ptr(rays), ptr(occ), len(rays), threads)
+        return occ
+
+    def close(self):
+        if self.rd:
+            self.L.ref_scene_free(C.c_void_p(self.rd))
+            self.rd = None
