@@ -90,6 +90,21 @@ int          rend_hip_set_mode(int mode);
  * and reports the rays traced; returns an error code instead of 0. */
 int          rend_hip_render_frame(struct rdata *rd, unsigned int *rays);
 
+/* Jittered camera rays of rd->cam for a w x h pixel grid, spp consecutive
+ * rays per pixel (pixel-major), written to d_rays (w*h*spp records). */
+int          rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays,
+                                  unsigned int w, unsigned int h,
+                                  unsigned int spp, unsigned int seed,
+                                  void *stream);
+
+/* One cosine-weighted diffuse ray per hit of a previous trace, about the
+ * geometric normal facing the incoming ray, origin p + 1e-3 * dir; a missed
+ * input gives a ray with tmax = -1. */
+int          rend_hip_gen_diffuse(struct rdata *rd, const struct rray *d_prim,
+                                  const struct rhit *d_hits,
+                                  struct rray *d_out, unsigned int n,
+                                  unsigned int seed, void *stream);
+
 /* Copies the device accumulator into rd->acc (kept on the device between
  * frames; rd->buf is refreshed after every frame). */
 int          rend_hip_download_acc(struct rdata *rd);

@@ -82,6 +82,10 @@ _SIGS = {
                                          C.c_uint]),
     "rend_hip_render_frame": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint)]),
     "rend_hip_download_acc": (C.c_int, [C.c_void_p]),
+    "rend_hip_gen_camrays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
+                                       C.c_uint, C.c_uint, C.c_void_p]),
+    "rend_hip_gen_diffuse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_uint, C.c_uint, C.c_void_p]),
     "rend_hip_stream": (C.c_void_p, []),
     "rend_hip_malloc": (C.c_void_p, [C.c_ulonglong]),
     "rend_hip_free": (C.c_int, [C.c_void_p]),

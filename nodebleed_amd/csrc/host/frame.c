@@ -30,6 +30,9 @@ int rk_shade_normals(const void *hits, const void *insts, const void *nrms,
                      float invsamples, float *acc, unsigned *buf,
                      void *stream);
 int rk_fill_f32(float *p, unsigned long long n, float v, void *stream);
+int rk_diffuse_rays(const void *prim, const void *hits, const void *insts,
+                    const void *tris, unsigned n, unsigned seed, void *out,
+                    void *stream);
 
 struct rl_frame {
 	void    *rays, *hits;
@@ -229,6 +232,37 @@ int rend_hip_render_frame(struct rdata *rd, unsigned *rays)
 	if (rays)
 		*rays = n;
 	return 0;
+}
+
+int rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays, unsigned int w,
+                         unsigned int h, unsigned int spp, unsigned int seed,
+                         void *stream)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	struct CamArgs ca;
+	rl_camargs(&rd->cam, &ca);
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	r = rk_camrays(&ca, w, h, spp ? spp : 1, seed, 0, d_rays, st);
+	return r ? rl_hipfail(r, "camera rays") : 0;
+}
+
+int rend_hip_gen_diffuse(struct rdata *rd, const struct rray *d_prim,
+                         const struct rhit *d_hits, struct rray *d_out,
+                         unsigned int n, unsigned int seed, void *stream)
+{
+	struct rl_scene *s = rl_scene(rd);
+	int r = rl_device();
+	if (r)
+		return r;
+	if (!s || !s->tlas_built)
+		return rl_fail(REND_ESTATE, "scene not prepared");
+	if ((r = rl_sync_scene(s)))
+		return r;
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	r = rk_diffuse_rays(d_prim, d_hits, s->insts, s->tris, n, seed, d_out, st);
+	return r ? rl_hipfail(r, "diffuse rays") : 0;
 }
 
 int rend_hip_download_acc(struct rdata *rd)

@@ -43,8 +43,8 @@ struct rl_scene {
 	void                *insts;      /* device dinst[] */
 	size_t               insts_cap;
 	unsigned             ninsts;
-	void                *nrms, *mtls;
-	size_t               nrms_cap, mtls_cap;
+	void                *nrms, *mtls, *tris;
+	size_t               nrms_cap, mtls_cap, tris_cap;
 	float               *acc;        /* device accumulator, 3 floats/pixel */
 	unsigned            *buf;        /* device ARGB8888 */
 	size_t               acc_cap;

@@ -200,6 +200,7 @@ static void scene_del(struct rdata *rd)
 			dfree((void **)&s->insts);
 			dfree((void **)&s->nrms);
 			dfree((void **)&s->mtls);
+			dfree((void **)&s->tris);
 			dfree((void **)&s->acc);
 			dfree((void **)&s->buf);
 			rl_frame_release(s);
@@ -436,7 +437,16 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 			return rl_hipfail(e, "materials");
 		s->mtls_cap = mb;
 	}
-	if ((nb && (e = hipMemcpy(s->nrms, rd->nrms, nb, hipMemcpyHostToDevice))
+	size_t tb = (size_t)s->maxtris * sizeof(struct rtri);
+	if (tb > s->tris_cap) {
+		dfree((void **)&s->tris);
+		if ((e = hipMalloc((void **)&s->tris, tb)) != hipSuccess)
+			return rl_hipfail(e, "triangles");
+		s->tris_cap = tb;
+	}
+	if ((tb && (e = hipMemcpy(s->tris, rd->tris, tb, hipMemcpyHostToDevice))
+	      != hipSuccess) ||
+	    (nb && (e = hipMemcpy(s->nrms, rd->nrms, nb, hipMemcpyHostToDevice))
 	      != hipSuccess) ||
 	    (mb && (e = hipMemcpy(s->mtls, rd->mtls, mb, hipMemcpyHostToDevice))
 	      != hipSuccess))

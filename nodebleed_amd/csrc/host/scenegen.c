@@ -846,8 +846,9 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 	float t[3] = {0, 0, 0}, q[4] = {0, 0, 0, 1}, sc[3] = {1, 1, 1};
 	if (mesh < 0 || rs_instance(s, (unsigned)mesh, t, q, sc, 0) < 0)
 		return -1;
-	float eye[3] = {0.0f, 0.85f, 0.3f}, tg[3] = {0.0f, 0.0f, -0.12f};
-	rs_camera(s, eye, tg, 38.0f, 1.0f, 0.0f);
+	/* Sees only terrain at 16:9 (every primary ray hits). */
+	float eye[3] = {0.0f, 0.95f, 0.25f}, tg[3] = {0.0f, 0.0f, -0.05f};
+	rs_camera(s, eye, tg, 34.0f, 1.0f, 0.0f);
 	rs_background(s, lights ? 0.0f : 0.4f, lights ? 0.0f : 0.4f,
 	  lights ? 0.0f : 0.4f);
 	return 0;

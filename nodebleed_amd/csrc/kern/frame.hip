@@ -142,6 +142,65 @@ __global__ void k_shade_normals(const rhit *__restrict__ hits,
            pack8(ag * invsamples) << 8 | pack8(ab * invsamples);
 }
 
+struct RTri {  // rend.h:64-71
+  float v0[3], p0, v1[3], p1, v2[3], p2;
+};
+
+// One cosine-weighted diffuse ray per primary hit, about the geometric
+// normal turned towards the incoming ray, origin p + 1e-3 * dir (EPS2,
+// util.h:8; rend.c:3281).  Basis: branchless ONB of Duff et al. 2017 as in
+// create_onb (rend.c:3126-3133); direction: rand_hemicos (rend.c:3135-3140).
+// A missed primary yields a ray with tmax = -1, which misses at the root.
+__global__ void k_diffuse_rays(const rray *__restrict__ prim,
+                               const rhit *__restrict__ hits,
+                               const dinst *__restrict__ insts,
+                               const RTri *__restrict__ tris, uint32_t n,
+                               uint32_t seed, rray *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const rray r = prim[i];
+  const rhit h = hits[i];
+  rray o;
+  o.pad = 0.0f;
+  if (h.id == 0xffffffffu) {
+    o.ox = r.ox; o.oy = r.oy; o.oz = r.oz; o.tmax = -1.0f;
+    o.dx = r.dx; o.dy = r.dy; o.dz = r.dz;
+  } else {
+    const dinst &in = insts[h.id & 0xfffu];
+    const RTri &t = tris[in.triofs + (h.id >> 12)];
+    const float e0x = t.v1[0] - t.v0[0], e0y = t.v1[1] - t.v0[1], e0z = t.v1[2] - t.v0[2];
+    const float e1x = t.v2[0] - t.v0[0], e1y = t.v2[1] - t.v0[1], e1z = t.v2[2] - t.v0[2];
+    float nx = e0y * e1z - e0z * e1y, ny = e0z * e1x - e0x * e1z, nz = e0x * e1y - e0y * e1x;
+    const float *m = in.inv;  // normal to world: transpose of the inverse
+    float wx = m[0] * nx + m[4] * ny + m[8] * nz;
+    float wy = m[1] * nx + m[5] * ny + m[9] * nz;
+    float wz = m[2] * nx + m[6] * ny + m[10] * nz;
+    const float il = 1.0f / sqrtf(wx * wx + wy * wy + wz * wz);
+    wx *= il; wy *= il; wz *= il;
+    if (wx * r.dx + wy * r.dy + wz * r.dz > 0.0f) { wx = -wx; wy = -wy; wz = -wz; }
+    const uint32_t key = rng_key(seed, 0x5bd1e995u, i);
+    const float u0 = rng_u01(key, 0), u1 = rng_u01(key, 1);
+    const float sgn = copysignf(1.0f, wz);
+    const float a = -1.0f / (sgn + wz);
+    const float b = wx * wy * a;
+    const float b1x = 1.0f + sgn * wx * wx * a, b1y = sgn * b, b1z = -sgn * wx;
+    const float b2x = b, b2y = sgn + wy * wy * a, b2z = -wy;
+    float s, c;
+    sincosf(6.28318531f * u0, &s, &c);
+    const float su = sqrtf(u1), lx = c * su, ly = s * su, lz = sqrtf(1.0f - u1);
+    const float dx = b1x * lx + b2x * ly + wx * lz;
+    const float dy = b1y * lx + b2y * ly + wy * lz;
+    const float dz = b1z * lx + b2z * ly + wz * lz;
+    const float px = r.ox + r.dx * h.t, py = r.oy + r.dy * h.t, pz = r.oz + r.dz * h.t;
+    o.ox = px + dx * 1e-3f; o.oy = py + dy * 1e-3f; o.oz = pz + dz * 1e-3f;
+    o.tmax = 3.402823466e+38f;
+    o.dx = dx; o.dy = dy; o.dz = dz;
+  }
+  float4 *po = (float4 *)&out[i];
+  po[0] = make_float4(o.ox, o.oy, o.oz, o.tmax);
+  po[1] = make_float4(o.dx, o.dy, o.dz, o.pad);
+}
+
 __global__ void k_fill_f32(float *p, size_t n, float v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -170,6 +229,16 @@ extern "C" int rk_shade_normals(const void *hits, const void *insts,
                      (hipStream_t)stream, (const rhit *)hits, (const dinst *)insts,
                      (const RNrm *)nrms, (const RMtl *)mtls, bg[0], bg[1], bg[2],
                      rw, rh, width, spp, invsamples, acc, buf);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_diffuse_rays(const void *prim, const void *hits,
+                               const void *insts, const void *tris, unsigned n,
+                               unsigned seed, void *out, void *stream) {
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_diffuse_rays, dim3((n + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, (const rray *)prim, (const rhit *)hits,
+                     (const dinst *)insts, (const RTri *)tris, n, seed, (rray *)out);
   return (int)hipGetLastError();
 }
 

@@ -207,6 +207,11 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       const uint32_t hitcnt = __builtin_popcount(hmr);
       if (hitcnt == 0) {
         need_pop = true;
+      } else if (sp + hitcnt > REND_STACK_MAX) {
+        // The reference asserts here (rend.c:834); report instead of writing
+        // past the stack.
+        atomicOr(&ctl[1], RK_ERR_STACK);
+        done = true;
       } else {
         // Highest rank among hits: nearest (closest hit) or, for the any-hit
         // pass, the reference's unordered push makes the highest child index
@@ -247,13 +252,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
             }
           }
         }
-        if (sp + hitcnt > REND_STACK_MAX) {
-          atomicOr(&ctl[1], RK_ERR_STACK);
-          done = true;
-        } else {
-          sp += hitcnt - 1;
-          ofs = next;
-        }
+        sp += hitcnt - 1;
+        ofs = next;
       }
     } else if (in_blas) {
       const dleaf *lf = (const dleaf *)(arena + base + (ofs & ~DNODE_LEAF));
