@@ -86,6 +86,68 @@ long long    rend_hip_arena_bytes(struct rdata *rd);
 #define REND_MODE_PATHTRACE  1
 int          rend_hip_set_mode(int mode);
 
+/* Path-tracing parameters: maximum segments per path, first depth at which
+ * Russian roulette applies (0 = never) and paths in flight per wavefront
+ * batch (0 = automatic). */
+int          rend_hip_pt_config(unsigned int maxdepth, unsigned int rr_depth,
+                                unsigned int batch);
+
+/* One view of a batch: camera looking from eye at target (upd_rcam
+ * semantics, main.c:159-195), resolution and samples per pixel. */
+struct rend_view {
+	float         eye[3], target[3];
+	float         vfov, focdist, focangle;
+	unsigned int  w, h, spp;
+};
+
+/* Path traces nviews views of rd's scene in one wavefront launch sequence
+ * (captured once as a HIP graph and replayed while the view table is
+ * unchanged).  d_acc holds sum(w*h*3) floats, the views' accumulators in
+ * order (radiance sums; the caller clears them); sample_base offsets every
+ * view's sample index so successive calls draw fresh samples.  rays (may be
+ * NULL) receives extension + shadow rays traced. */
+int          rend_hip_render_views(struct rdata *rd, const struct rend_view *views,
+                                   unsigned int nviews, unsigned int sample_base,
+                                   float *d_acc, void *stream,
+                                   unsigned long long *rays);
+
+/* Times candidate launch shapes (paths in flight per batch, traversal
+ * refill threshold) on one call of rend_hip_render_views with the given
+ * views and keeps the fastest for later calls.  out (may be NULL) receives
+ * {batch, refill, ms of the best, ms of the default}. */
+int          rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
+                               unsigned int nviews, float *d_acc, float out[4]);
+
+/* HIP-graph capture of path-tracing frames (default on; guided frames are
+ * always launched directly). */
+int          rend_hip_pt_graphs(int on);
+
+/* ---- learned path guiding (no counterpart in the reference) -------------
+ * A hash-grid-encoded 3x64 MLP outputs an 8-lobe vMF mixture per shading
+ * point; the indirect direction is drawn from mix * mixture + (1 - mix) *
+ * cosine BSDF (one-sample MIS, unbiased for 0 <= mix < 1).  The network is
+ * trained online from the vertices of every train_stride-th path, `steps`
+ * Adam steps of 32K records per wavefront batch, learning rate lr (hash
+ * grid; MLP lr / 10).  The proposal is used from the first trained batch on. */
+int          rend_hip_guide_enable(struct rdata *rd, float mix,
+                                   unsigned int train_stride, unsigned int steps,
+                                   float lr, unsigned int seed);
+int          rend_hip_guide_disable(struct rdata *rd);
+/* Changes mix and training steps (0 = frozen network) while enabled. */
+int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps);
+/* out: last training loss, records of the last batch, Adam steps, enabled. */
+int          rend_hip_guide_info(struct rdata *rd, float out[4]);
+/* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),
+ * raw network outputs for positions+normals (6 floats each, 64 outputs per
+ * point), and the loss gradient over training records. */
+int          rend_hip_guide_params(struct rdata *rd, float *params, int set);
+int          rend_hip_guide_forward_host(struct rdata *rd, const float *pn,
+                                         unsigned int n, float *out);
+int          rend_hip_guide_grad_host(struct rdata *rd, const void *records,
+                                      unsigned int n, float *grad, float *loss);
+/* D = A (16x32) * B (32x16), row-major fp32 in, bf16 MFMA inside. */
+int          rend_hip_mfma_check(const float *A, const float *B, float *D);
+
 /* Renders one frame of rd (what rend_render does for the claiming thread)
  * and reports the rays traced; returns an error code instead of 0. */
 int          rend_hip_render_frame(struct rdata *rd, unsigned int *rays);

@@ -97,6 +97,10 @@ struct rs_view {
 	float          bgcol[3];
 };
 void rs_rdata_view(const struct rdata *rd, struct rs_view *v);
+/* Camera constants of rd->cam: eye, ri, up, fwd (3 each), 1/w, 1/h, aspect,
+ * focus distance, 2 tan(vfov/2) focdist, lens radius (lane 0 of the vector
+ * fields upd_rcam fills, main.c:159-195). */
+void rs_rdata_camera(const struct rdata *rd, float out[18]);
 /* Sets the frame-protocol fields the reference's main loop writes
  * (main.c:424-426) and the application colour buffer. */
 void rs_rdata_setframe(struct rdata *rd, int blknum, unsigned int rays,

@@ -69,6 +69,7 @@ _SIGS = {
     "rend_hip_set_rcp": (C.c_int, [C.c_int]),
     "rend_hip_set_tuning": (C.c_int, [C.c_uint, C.c_uint]),
     "rend_hip_set_mode": (C.c_int, [C.c_int]),
+    "rend_hip_pt_config": (C.c_int, [C.c_uint, C.c_uint, C.c_uint]),
     "rend_hip_upload": (C.c_int, [C.c_void_p]),
     "rend_hip_hazards": (C.c_uint, [C.c_void_p]),
     "rend_hip_arena_bytes": (C.c_longlong, [C.c_void_p]),
@@ -96,6 +97,22 @@ _SIGS = {
     "rend_hip_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rend_hip_event_elapsed": (C.c_float, [C.c_void_p, C.c_void_p]),
     "rend_hip_event_destroy": (C.c_int, [C.c_void_p]),
+    "rend_hip_pt_graphs": (C.c_int, [C.c_int]),
+    "rend_hip_render_views": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
+                                        C.c_void_p, C.c_void_p, C.POINTER(C.c_ulonglong)]),
+    "rend_hip_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p,
+                                    C.POINTER(C.c_float)]),
+    "rend_hip_guide_enable": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.c_uint,
+                                        C.c_float, C.c_uint]),
+    "rend_hip_guide_disable": (C.c_int, [C.c_void_p]),
+    "rend_hip_guide_set": (C.c_int, [C.c_void_p, C.c_float, C.c_uint]),
+    "rend_hip_guide_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "rend_hip_guide_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
+    "rend_hip_guide_forward_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint,
+                                              C.c_void_p]),
+    "rend_hip_guide_grad_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p,
+                                           C.POINTER(C.c_float)]),
+    "rend_hip_mfma_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rs_new": (C.c_void_p, []),
     "rs_free": (None, [C.c_void_p]),
     "rs_config": (C.c_int, [C.c_void_p, C.c_int, C.c_uint, C.c_float]),
@@ -124,6 +141,7 @@ _SIGS = {
     "rs_instance_glob": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_float)]),
     "rs_rdata_view": (None, [C.c_void_p, C.POINTER(RsView)]),
     "rs_rdata_setframe": (None, [C.c_void_p, C.c_int, C.c_uint, C.c_uint, C.c_void_p]),
+    "rs_rdata_camera": (None, [C.c_void_p, C.POINTER(C.c_float)]),
     "rs_rdata_new": (C.c_void_p, []),
     "rs_rdata_free": (None, [C.c_void_p]),
     "rs_sizeof_rdata": (C.c_uint, []),
@@ -244,6 +262,36 @@ class Scene:
         check(self.L.rend_hip_occluded_host(self.rd, ptr(rays), ptr(occ), len(rays)),
               "occluded")
         return occ
+
+    def camera(self) -> np.ndarray:
+        """eye, ri, up, fwd, 1/w, 1/h, aspect, focdist, fovfdist, focrad."""
+        out = np.zeros(18, np.float32)
+        self.L.rs_rdata_camera(self.rd, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
+
+    def render(self, frames: int = 1, mode: int = REND_MODE_PATHTRACE, maxdepth: int = 4,
+               rr_depth: int = 3, batch: int = 0, block=(1, 1)) -> tuple[np.ndarray, int]:
+        """Renders `frames` frames through the frame protocol of rend_render
+        and returns (mean radiance image h x w x 3, rays traced)."""
+        L = self.L
+        v = self.view()
+        w, h = v.width, v.height
+        L.rend_staticinit(*block)
+        check(L.rend_hip_set_mode(mode), "mode")
+        check(L.rend_hip_pt_config(maxdepth, rr_depth, batch), "pt config")
+        L.rend_clraccum(self.rd)
+        buf = np.zeros(w * h, np.uint32)
+        rays, samples = 0, 0
+        for _ in range(frames):
+            L.rs_rdata_setframe(self.rd, 0, 0, samples, buf.ctypes.data)
+            n = C.c_uint(0)
+            check(L.rend_hip_render_frame(self.rd, C.byref(n)), "render")
+            rays += n.value
+            samples += v.spp
+        check(L.rend_hip_download_acc(self.rd), "download")
+        acc = self._arr(self.view().acc, np.float32, w * h * 3).reshape(h, w, 3)
+        self.argb = buf.reshape(h, w)
+        return acc / max(samples, 1), rays
 
     def close(self):
         if self.rd:

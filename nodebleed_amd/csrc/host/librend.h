@@ -30,6 +30,21 @@ struct rl_global {
 };
 
 struct rl_frame;  /* frame.c */
+struct rl_pt;     /* pt.c */
+
+/* Learned proposal state (guide.c). */
+struct rl_guide {
+	float    *master, *grad, *m1, *m2;  /* fp32 parameters, gradient, Adam moments */
+	void     *grid16, *w16;             /* fp16 grid / bf16 weight mirrors */
+	float    *mix;                      /* per queued path mixture */
+	void     *rec;                      /* training records of a batch */
+	float    *scal;                     /* [0] weight sum, [1] loss */
+	size_t    cap;
+	float     gmin[3], ginv[3];
+	float     mix_prob, lr_grid, lr_mlp;
+	unsigned  stride, steps, step, last_records;
+	int       enabled;
+};
 
 struct rl_scene {
 	struct rdata        *rd;
@@ -53,6 +68,8 @@ struct rl_scene {
 	int                  static_dirty, dynamic_dirty, tlas_built;
 	unsigned             hazards;
 	struct rl_frame     *frame;
+	struct rl_pt        *pt;
+	struct rl_guide     *guide;
 };
 
 extern struct rl_global G;
@@ -69,6 +86,17 @@ int              rl_check_errors(hipStream_t st);
 unsigned         rl_blkszx(void);
 unsigned         rl_blkszy(void);
 void             rl_frame_release(struct rl_scene *s);
+void             rl_pt_release(struct rl_scene *s);
+void             rl_guide_release(struct rl_scene *s);
+struct PtArgs;
+void             rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap);
+int              rl_guide_buffers(struct rl_scene *s, size_t cap);
+int              rl_guide_eval(struct rl_scene *s, const struct PtArgs *a,
+                               const void *rays, const void *hits,
+                               const unsigned *counts, hipStream_t st);
+int              rl_guide_train(struct rl_scene *s, const struct PtArgs *a,
+                                const void *paths, unsigned *counts,
+                                hipStream_t st);
 int              rl_sync_scene(struct rl_scene *s);
 int              rl_render_pt(struct rl_scene *s, unsigned rw, unsigned rh,
                               unsigned *rays);
@@ -82,6 +110,7 @@ int rk_trace_launch(const void *arena, unsigned long long tlas,
                     void *occ, unsigned int n, unsigned int *ctl,
                     void *spill, unsigned long long *stats, int closest,
                     int count, int rcp_ieee, unsigned int grid,
-                    unsigned int refill, unsigned int max_iter, void *stream);
+                    unsigned int refill, unsigned int max_iter,
+                    const unsigned int *n_dev, void *stream);
 
 #endif

@@ -204,6 +204,7 @@ static void scene_del(struct rdata *rd)
 			dfree((void **)&s->acc);
 			dfree((void **)&s->buf);
 			rl_frame_release(s);
+			rl_pt_release(s);
 			free(s->blas_triofs);
 			free(s->blas_dev);
 			free(s);
@@ -677,7 +678,7 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
 	  d_occ, n, G.ctl + 2 * slot, G.spill, count ? G.stats : NULL,
-	  closest, count, G.rcp_ieee, grid, G.refill, G.max_iter, st);
+	  closest, count, G.rcp_ieee, grid, G.refill, G.max_iter, NULL, st);
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;

@@ -964,6 +964,22 @@ void rs_rdata_view(const struct rdata *rd, struct rs_view *v)
 	v->bgcol[2] = rd->bgcol.z;
 }
 
+void rs_rdata_camera(const struct rdata *rd, float out[18])
+{
+	struct CamArgs a;
+	rl_camargs(&rd->cam, &a);
+	memcpy(out, a.eye, 3 * sizeof(float));
+	memcpy(out + 3, a.ri, 3 * sizeof(float));
+	memcpy(out + 6, a.up, 3 * sizeof(float));
+	memcpy(out + 9, a.fwd, 3 * sizeof(float));
+	out[12] = a.rw;
+	out[13] = a.rh;
+	out[14] = a.aspect;
+	out[15] = a.fdist;
+	out[16] = a.fovfdist;
+	out[17] = a.focrad;
+}
+
 void rs_rdata_setframe(struct rdata *rd, int blknum, unsigned int rays,
                        unsigned int samples, unsigned int *buf)
 {

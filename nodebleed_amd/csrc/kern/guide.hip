@@ -1,0 +1,542 @@
+// Learned path-guiding proposal: inference, training and optimiser kernels.
+//
+// Model: multiresolution hash encoding of the shading position (Mueller,
+// Evans, Schied, Keller 2022, "Instant Neural Graphics Primitives": GD_LEVELS
+// levels with geometric resolutions GD_NMIN..GD_NMAX, dense indexing while a
+// level fits the table, spatial hash x ^ 2654435761 y ^ 805459861 z
+// otherwise, trilinear interpolation), concatenated with the shading normal
+// and a constant 1, through a 3 x 64 ReLU MLP whose outputs parameterise a
+// mixture of GD_LOBES von Mises-Fisher lobes over the sphere.
+//
+// MLP layers run on MFMA in BF16 with FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16): one wave owns 64 samples, activations stay in
+// LDS between layers, weights are read as MFMA B fragments from a BF16
+// mirror of the FP32 master parameters.  Training minimises the Monte Carlo
+// estimate of the KL divergence between the normalised product
+// L_i(w) max(0, n.w) and the mixture, -(L cos / q) log p(w), with gradients
+// through the mixture, the MLP (MFMA) and the hash grid (atomic scatter),
+// and Adam on the FP32 master copy, which then refreshes the FP16 grid and
+// BF16 weight mirrors.  vMF sampling follows Jakob 2012 ("Numerically stable
+// sampling of the von Mises Fisher distribution on S^2").
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "../ptlayout.h"
+#include "rend_rays.h"
+#include "guide_math.h"
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__constant__ float c_gd_res[GD_LEVELS];      // grid resolution per level
+__constant__ uint32_t c_gd_dense[GD_LEVELS];  // 1 when (res+1)^3 <= GD_TABLE
+
+namespace {
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Index and weight of corner c of level l for a normalised position.
+__device__ __forceinline__ void grid_corner(int l, const float p[3], int c, uint32_t &idx,
+                                            float &w) {
+  const float res = c_gd_res[l];
+  float wt = 1.0f;
+  uint32_t ix[3];
+  for (int a = 0; a < 3; a++) {
+    const float x = p[a] * res;
+    const float f = floorf(x);
+    const float fr = x - f;
+    const uint32_t bit = (c >> a) & 1u;
+    ix[a] = (uint32_t)f + bit;
+    wt *= bit ? fr : 1.0f - fr;
+  }
+  if (c_gd_dense[l]) {
+    const uint32_t r1 = (uint32_t)res + 1u;
+    idx = ix[0] + ix[1] * r1 + ix[2] * r1 * r1;
+  } else {
+    idx = (ix[0] ^ (ix[1] * 2654435761u) ^ (ix[2] * 805459861u)) & (GD_TABLE - 1u);
+  }
+  w = wt;
+}
+
+__device__ __forceinline__ void normalise_pos(const PtArgs &a, const float p[3], float q[3]) {
+  for (int k = 0; k < 3; k++) {
+    float v = (p[k] - a.gmin[k]) * a.ginv[k];
+    q[k] = fminf(fmaxf(v, 0.0f), 0.99999f);
+  }
+}
+
+// Input row (GD_IN values) of one sample into LDS as bf16.
+__device__ __forceinline__ void encode(const PtArgs &a, const __half *__restrict__ grid,
+                                       const float p[3], const float n[3], __bf16 *row) {
+  float q[3];
+  normalise_pos(a, p, q);
+  for (int l = 0; l < GD_LEVELS; l++) {
+    float f0 = 0.0f, f1 = 0.0f;
+    for (int c = 0; c < 8; c++) {
+      uint32_t idx;
+      float w;
+      grid_corner(l, q, c, idx, w);
+      const __half2 g = *(const __half2 *)&grid[((size_t)l * GD_TABLE + idx) * GD_FEAT];
+      f0 += w * __low2float(g);
+      f1 += w * __high2float(g);
+    }
+    row[2 * l] = (__bf16)f0;
+    row[2 * l + 1] = (__bf16)f1;
+  }
+  row[24] = (__bf16)n[0];
+  row[25] = (__bf16)n[1];
+  row[26] = (__bf16)n[2];
+  row[27] = (__bf16)1.0f;
+  for (int k = 28; k < GD_IN; k++) row[k] = (__bf16)0.0f;
+}
+
+// Gathers 8 bf16 at base[0], base[s], ..., base[7s].
+__device__ __forceinline__ bf16x8 frag(const __bf16 *base, int s) {
+  bf16x8 v;
+  if (s == 1) return *(const bf16x8 *)base;
+#pragma unroll
+  for (int j = 0; j < 8; j++) v[j] = base[j * s];
+  return v;
+}
+
+// D (16x16 tile at rows r0.., cols n0..) += A[rows][k] * B[k][cols] over K,
+// with A(i,k) = A[i*ars + k*acs] and B(k,n) = B[k*bks + n*bns].
+__device__ __forceinline__ f32x4 tile_mm(const __bf16 *A, int ars, int acs, const __bf16 *B,
+                                         int bks, int bns, int r0, int n0, int K) {
+  const uint32_t l = lane_id();
+  const int i = r0 + (int)(l & 15u), n = n0 + (int)(l & 15u), kq = 8 * (int)(l >> 4);
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bf16x8 av = frag(A + (size_t)i * ars + (size_t)(k0 + kq) * acs, acs);
+    const bf16x8 bv = frag(B + (size_t)(k0 + kq) * bks + (size_t)n * bns, bks);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// One dense layer for the wave's 64 rows: out[m][n] = act(in[m][:] . W[n][:] + b[n]).
+// W is out-major (W[n][k]); out rows hold `nout` columns.
+template <bool RELU>
+__device__ __forceinline__ void layer(const __bf16 *in, int kin, const __bf16 *__restrict__ W,
+                                      const float *__restrict__ b, int nout, __bf16 *out,
+                                      float *outf) {
+  const uint32_t l = lane_id();
+  for (int rt = 0; rt < kWave; rt += 16)
+    for (int ct = 0; ct < nout; ct += 16) {
+      const f32x4 d = tile_mm(in, kin, 1, W, 1, kin, rt, ct, kin);
+      const int n = ct + (int)(l & 15u);
+      const float bn = b[n];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = rt + 4 * (int)(l >> 4) + r;
+        float v = d[r] + bn;
+        if (RELU) v = fmaxf(v, 0.0f);
+        if (out) out[m * nout + n] = (__bf16)v;
+        if (outf) outf[m * nout + n] = v;
+      }
+    }
+}
+
+struct Net {
+  const __half *grid;   // fp16 mirror [levels][table][feat]
+  const __bf16 *w;      // bf16 mirror of W1..W4 (out-major, packed)
+  const float *master;  // fp32 master (biases read from here)
+};
+
+__device__ __forceinline__ const __bf16 *wptr(const Net &N, int layer_id) {
+  switch (layer_id) {
+    case 0: return N.w;
+    case 1: return N.w + GD_HID * GD_IN;
+    case 2: return N.w + GD_HID * GD_IN + GD_HID * GD_HID;
+    default: return N.w + GD_HID * GD_IN + 2 * GD_HID * GD_HID;
+  }
+}
+
+// Forward for the wave: inputs already in act0; leaves hidden activations in
+// act1..act3 and raw outputs (fp32) in outf.
+__device__ __forceinline__ void forward(const Net &N, __bf16 *act0, __bf16 *act1, __bf16 *act2,
+                                        __bf16 *act3, float *outf) {
+  layer<true>(act0, GD_IN, wptr(N, 0), N.master + GD_OFS_B1, GD_HID, act1, nullptr);
+  __syncthreads();
+  layer<true>(act1, GD_HID, wptr(N, 1), N.master + GD_OFS_B2, GD_HID, act2, nullptr);
+  __syncthreads();
+  layer<true>(act2, GD_HID, wptr(N, 2), N.master + GD_OFS_B3, GD_HID, act3, nullptr);
+  __syncthreads();
+  layer<false>(act3, GD_HID, wptr(N, 3), N.master + GD_OFS_B4, GD_OUT, nullptr, outf);
+  __syncthreads();
+}
+
+// Geometric hit point and facing normal of queue entry i (as in k_pt_shade).
+struct RTri {
+  float v0[3], p0, v1[3], p1, v2[3], p2;
+};
+
+__device__ __forceinline__ bool hit_point(const rray &r, const rhit &h,
+                                          const dinst *__restrict__ insts,
+                                          const RTri *__restrict__ tris, float p[3], float n[3]) {
+  if (h.id == 0xffffffffu) return false;
+  const dinst &in = insts[h.id & 0xfffu];
+  const RTri &T = tris[in.triofs + (h.id >> 12)];
+  const float e0x = T.v1[0] - T.v0[0], e0y = T.v1[1] - T.v0[1], e0z = T.v1[2] - T.v0[2];
+  const float e1x = T.v2[0] - T.v0[0], e1y = T.v2[1] - T.v0[1], e1z = T.v2[2] - T.v0[2];
+  const float gx0 = e0y * e1z - e0z * e1y, gy0 = e0z * e1x - e0x * e1z, gz0 = e0x * e1y - e0y * e1x;
+  const float *m = in.inv;
+  float gx = m[0] * gx0 + m[4] * gy0 + m[8] * gz0;
+  float gy = m[1] * gx0 + m[5] * gy0 + m[9] * gz0;
+  float gz = m[2] * gx0 + m[6] * gy0 + m[10] * gz0;
+  const float il = 1.0f / sqrtf(gx * gx + gy * gy + gz * gz);
+  gx *= il; gy *= il; gz *= il;
+  if (gx * r.dx + gy * r.dy + gz * r.dz > 0.0f) { gx = -gx; gy = -gy; gz = -gz; }
+  p[0] = r.ox + r.dx * h.t; p[1] = r.oy + r.dy * h.t; p[2] = r.oz + r.dz * h.t;
+  n[0] = gx; n[1] = gy; n[2] = gz;
+  return true;
+}
+
+// LDS per wave: act0 (64 x 32) + act1..3 (64 x 64) bf16, outputs 64 x 64 fp32.
+struct WaveLds {
+  __attribute__((aligned(16))) __bf16 act0[kWave * GD_IN];
+  __attribute__((aligned(16))) __bf16 act1[kWave * GD_HID];
+  __attribute__((aligned(16))) __bf16 act2[kWave * GD_HID];
+  __attribute__((aligned(16))) __bf16 act3[kWave * GD_HID];
+  __attribute__((aligned(16))) float out[kWave * GD_OUT];
+};
+
+// Mixture of every queued hit point: mix[i][GD_MIX].
+__global__ __launch_bounds__(64) void k_guide_eval(PtArgs a, Net N,
+                                                   const rray *__restrict__ rays,
+                                                   const rhit *__restrict__ hits,
+                                                   const uint32_t *__restrict__ counts,
+                                                   const dinst *__restrict__ insts,
+                                                   const RTri *__restrict__ tris,
+                                                   float *__restrict__ mix) {
+  __shared__ WaveLds S;
+  const uint32_t n = counts[PT_CNT_CUR];
+  const uint32_t lane = lane_id();
+  for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
+    const uint32_t i = base + lane;
+    float p[3] = {0, 0, 0}, nn[3] = {0, 0, 1};
+    bool ok = false;
+    if (i < n) ok = hit_point(rays[i], hits[i], insts, tris, p, nn);
+    encode(a, N.grid, p, nn, &S.act0[lane * GD_IN]);
+    __syncthreads();
+    forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
+    if (i < n && ok) {
+      float m[GD_MIX];
+      gd_mixture(&S.out[lane * GD_OUT], m);
+      float4 *dst = (float4 *)&mix[(size_t)i * GD_MIX];
+#pragma unroll
+      for (int k = 0; k < GD_MIX / 4; k++)
+        dst[k] = make_float4(m[4 * k], m[4 * k + 1], m[4 * k + 2], m[4 * k + 3]);
+    }
+    __syncthreads();
+  }
+}
+
+// Incident-radiance targets of the frame's records from the final path
+// radiance; accumulates the sum of training weights into wsum.
+__global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__restrict__ counts,
+                                const PtPath *__restrict__ paths, float *__restrict__ wsum,
+                                uint32_t cap) {
+  const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
+  float acc = 0.0f;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    GdRecord &r = rec[i];
+    const PtPath &p = paths[r.path];
+    const float Lf = 0.2126f * p.L[0] + 0.7152f * p.L[1] + 0.0722f * p.L[2];
+    float li = r.thr > 0.0f ? fmaxf(Lf - r.lbefore, 0.0f) / r.thr : 0.0f;
+    if (!(li < 1e6f)) li = 0.0f;
+    const float c = fmaxf(r.nrm[0] * r.dir[0] + r.nrm[1] * r.dir[1] + r.nrm[2] * r.dir[2], 0.0f);
+    r.target = r.pdf > 0.0f ? li * c / r.pdf : 0.0f;
+    acc += r.target;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
+  if ((threadIdx.x & 63u) == 0 && acc > 0.0f) atomicAdd(wsum, acc);
+}
+
+// Forward + backward over records [first, first + n): gradients of the mean
+// of -(target / mean target) log p(dir) into grad (fp32, atomics).
+__global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
+                                                    const GdRecord *__restrict__ rec,
+                                                    const uint32_t *__restrict__ counts,
+                                                    uint32_t cap, uint32_t first,
+                                                    uint32_t chunk,
+                                                    const float *__restrict__ wsum,
+                                                    float *__restrict__ grad,
+                                                    float *__restrict__ loss) {
+  __shared__ WaveLds S;
+  __shared__ __attribute__((aligned(16))) __bf16 dout[kWave * GD_OUT];
+  __shared__ __attribute__((aligned(16))) __bf16 dh[kWave * GD_HID];
+  const uint32_t lane = lane_id();
+  // records of this step: [first, first + n) within the frame's valid records
+  const uint32_t total = counts ? min(counts[PT_CNT_TRAIN], cap) : cap;
+  const uint32_t n = first < total ? min(chunk, total - first) : 0u;
+  const float wmean = wsum[0] / (float)(total ? total : 1u);
+  const float inv = wmean > 0.0f ? 1.0f / (wmean * (float)(n ? n : 1u)) : 0.0f;
+  float lacc = 0.0f;
+  for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
+    const uint32_t i = base + lane;
+    const bool valid = i < n;
+    GdRecord r;
+    if (valid) r = rec[first + i];
+    else {
+      for (int k = 0; k < 3; k++) { r.pos[k] = 0.0f; r.nrm[k] = 0.0f; r.dir[k] = 0.0f; }
+      r.nrm[2] = 1.0f; r.dir[2] = 1.0f; r.target = 0.0f;
+    }
+    encode(a, N.grid, r.pos, r.nrm, &S.act0[lane * GD_IN]);
+    __syncthreads();
+    forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
+    // d loss / d raw outputs for this lane's sample
+    float g[GD_OUT];
+    const float wgt = valid ? r.target * inv : 0.0f;
+    lacc += gd_loss_grad(&S.out[lane * GD_OUT], r.dir, wgt, g);
+    for (int k = 0; k < GD_OUT; k++) dout[lane * GD_OUT + k] = (__bf16)g[k];
+    __syncthreads();
+    // bias gradients: column sums of dout
+    for (int k = 0; k < GD_OUT; k++) {
+      float v = g[k];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+      if (lane == 0 && v != 0.0f) atomicAdd(&grad[GD_OFS_B4 + k], v);
+    }
+    // Backward through the four layers.  For layer j with input X (64 x K),
+    // weight W (N x K, out-major) and output gradient D (64 x N):
+    //   dW[n][k] = sum_m D[m][n] X[m][k];  dX[m][k] = sum_n D[m][n] W[n][k].
+    const __bf16 *Xs[4];
+    Xs[0] = S.act0;
+    Xs[1] = S.act1;
+    Xs[2] = S.act2;
+    Xs[3] = S.act3;
+    const int Ks[4] = {GD_IN, GD_HID, GD_HID, GD_HID};
+    const int Ns[4] = {GD_HID, GD_HID, GD_HID, GD_OUT};
+    const uint32_t wofs[4] = {GD_OFS_W1, GD_OFS_W2, GD_OFS_W3, GD_OFS_W4};
+    const uint32_t bofs[4] = {GD_OFS_B1, GD_OFS_B2, GD_OFS_B3, GD_OFS_B4};
+    const __bf16 *D = dout;
+    for (int j = 3; j >= 0; j--) {
+      const int K = Ks[j], Nn = Ns[j];
+      // dW (N x K) = D^T X : A(n, m) = D[m*Nn + n], B(m, k) = X[m*K + k]
+      for (int rt = 0; rt < Nn; rt += 16)
+        for (int ct = 0; ct < K; ct += 16) {
+          const f32x4 d = tile_mm(D, 1, Nn, Xs[j], K, 1, rt, ct, kWave);
+          const int k = ct + (int)(lane & 15u);
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int nrow = rt + 4 * (int)(lane >> 4) + q;
+            if (d[q] != 0.0f) atomicAdd(&grad[wofs[j] + nrow * K + k], d[q]);
+          }
+        }
+      if (j == 0) {
+        // dX for the encoding inputs (first 24 columns)
+        for (int rt = 0; rt < kWave; rt += 16)
+          for (int ct = 0; ct < GD_IN; ct += 16) {
+            const f32x4 d = tile_mm(D, Nn, 1, wptr(N, 0), K, 1, rt, ct, Nn);
+            const int k = ct + (int)(lane & 15u);
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int m = rt + 4 * (int)(lane >> 4) + q;
+              S.out[m * GD_OUT + k] = d[q];  // reuse as fp32 scratch
+            }
+          }
+        __syncthreads();
+        // scatter into the hash grid
+        if (valid) {
+          float q3[3];
+          normalise_pos(a, r.pos, q3);
+          for (int l = 0; l < GD_LEVELS; l++) {
+            const float d0 = S.out[lane * GD_OUT + 2 * l], d1 = S.out[lane * GD_OUT + 2 * l + 1];
+            if (d0 == 0.0f && d1 == 0.0f) continue;
+            for (int c = 0; c < 8; c++) {
+              uint32_t idx;
+              float w;
+              grid_corner(l, q3, c, idx, w);
+              const size_t o = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
+              atomicAdd(&grad[o], w * d0);
+              atomicAdd(&grad[o + 1], w * d1);
+            }
+          }
+        }
+        __syncthreads();
+        break;
+      }
+      // dX (64 x K) = D W, then the ReLU mask of X (X = relu output)
+      __bf16 *nd = dh;
+      for (int rt = 0; rt < kWave; rt += 16)
+        for (int ct = 0; ct < K; ct += 16) {
+          const f32x4 d = tile_mm(D, Nn, 1, wptr(N, j), K, 1, rt, ct, Nn);
+          const int k = ct + (int)(lane & 15u);
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int m = rt + 4 * (int)(lane >> 4) + q;
+            const float x = (float)Xs[j][m * K + k];
+            S.out[m * GD_OUT + k] = x > 0.0f ? d[q] : 0.0f;
+          }
+        }
+      __syncthreads();
+      // bias gradient of the previous layer and bf16 copy for the next step
+      for (int k = 0; k < K; k++) {
+        float v = S.out[lane * GD_OUT + k];
+        nd[lane * K + k] = (__bf16)v;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+        if (lane == 0 && v != 0.0f) atomicAdd(&grad[bofs[j - 1] + k], v);
+      }
+      __syncthreads();
+      // move dh into dout so the next iteration reads it as D
+      for (int k = 0; k < K; k++) dout[lane * GD_HID + k] = nd[lane * K + k];
+      __syncthreads();
+      D = dout;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) lacc += __shfl_down(lacc, o);
+  if (lane == 0 && loss) atomicAdd(loss, lacc);
+}
+
+// Adam on the fp32 master parameters; refreshes the fp16/bf16 mirrors and
+// clears the gradient.
+__global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ grad,
+                             float *__restrict__ m1, float *__restrict__ m2,
+                             __half *__restrict__ grid16, __bf16 *__restrict__ w16,
+                             uint32_t step, float lr_grid, float lr_mlp) {
+  const float b1 = 0.9f, b2 = 0.99f, eps = 1e-10f;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < GD_NPARAM;
+       i += gridDim.x * blockDim.x) {
+    const float g = grad[i];
+    float p = master[i];
+    if (g != 0.0f || i >= GD_N_GRID) {
+      const float a = b1 * m1[i] + (1.0f - b1) * g;
+      const float b = b2 * m2[i] + (1.0f - b2) * g * g;
+      m1[i] = a;
+      m2[i] = b;
+      const float lr = i < GD_N_GRID ? lr_grid : lr_mlp;
+      p -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+      master[i] = p;
+      grad[i] = 0.0f;
+    }
+    if (i < GD_N_GRID) grid16[i] = __float2half(p);
+  }
+  // bf16 mirror of the weights (not biases), packed W1 W2 W3 W4
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < (uint32_t)GD_NMLP;
+       j += gridDim.x * blockDim.x) {
+    const uint32_t i = GD_N_GRID + j;
+    int dst = -1;
+    if (i >= GD_OFS_W1 && i < GD_OFS_B1) dst = (int)(i - GD_OFS_W1);
+    else if (i >= GD_OFS_W2 && i < GD_OFS_B2) dst = GD_HID * GD_IN + (int)(i - GD_OFS_W2);
+    else if (i >= GD_OFS_W3 && i < GD_OFS_B3) dst = GD_HID * GD_IN + GD_HID * GD_HID + (int)(i - GD_OFS_W3);
+    else if (i >= GD_OFS_W4 && i < GD_OFS_B4) dst = GD_HID * GD_IN + 2 * GD_HID * GD_HID + (int)(i - GD_OFS_W4);
+    if (dst >= 0) w16[dst] = (__bf16)master[i];
+  }
+}
+
+// Standalone forward for tests: positions/normals (6 floats each) -> raw outputs.
+__global__ __launch_bounds__(64) void k_guide_forward(PtArgs a, Net N, const float *__restrict__ pn,
+                                                      uint32_t n, float *__restrict__ out) {
+  __shared__ WaveLds S;
+  const uint32_t lane = lane_id();
+  for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
+    const uint32_t i = base + lane;
+    float p[3] = {0, 0, 0}, nn[3] = {0, 0, 1};
+    if (i < n) {
+      for (int k = 0; k < 3; k++) { p[k] = pn[6 * i + k]; nn[k] = pn[6 * i + 3 + k]; }
+    }
+    encode(a, N.grid, p, nn, &S.act0[lane * GD_IN]);
+    __syncthreads();
+    forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
+    if (i < n)
+      for (int k = 0; k < GD_OUT; k++) out[(size_t)i * GD_OUT + k] = S.out[lane * GD_OUT + k];
+    __syncthreads();
+  }
+}
+
+// MFMA lane-layout check for tests: D = A (16x32) * B (32x16), row-major.
+__global__ void k_mfma_check(const float *A, const float *B, float *D) {
+  __shared__ __attribute__((aligned(16))) __bf16 a[16 * 32];
+  __shared__ __attribute__((aligned(16))) __bf16 b[32 * 16];
+  const uint32_t l = lane_id();
+  for (int k = l; k < 16 * 32; k += 64) { a[k] = (__bf16)A[k]; b[k] = (__bf16)B[k]; }
+  __syncthreads();
+  const f32x4 d = tile_mm(a, 32, 1, b, 16, 1, 0, 0, 32);
+  for (int r = 0; r < 4; r++) D[(4 * (l >> 4) + r) * 16 + (l & 15u)] = d[r];
+}
+
+}  // namespace
+
+// ---- C-ABI --------------------------------------------------------------------
+
+extern "C" int rk_guide_setup(void *stream) {
+  float res[GD_LEVELS];
+  uint32_t dense[GD_LEVELS];
+  const double b = exp((log((double)GD_NMAX) - log((double)GD_NMIN)) / (GD_LEVELS - 1));
+  for (int l = 0; l < GD_LEVELS; l++) {
+    res[l] = (float)floor(GD_NMIN * pow(b, l));
+    const double r1 = res[l] + 1.0;
+    dense[l] = r1 * r1 * r1 <= (double)GD_TABLE;
+  }
+  hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_gd_res), res, sizeof(res), 0,
+                                        hipMemcpyHostToDevice, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_gd_dense), dense, sizeof(dense), 0,
+                               hipMemcpyHostToDevice, (hipStream_t)stream);
+  return (int)e;
+}
+
+static Net mknet(const void *grid16, const void *w16, const float *master) {
+  Net N;
+  N.grid = (const __half *)grid16;
+  N.w = (const __bf16 *)w16;
+  N.master = master;
+  return N;
+}
+
+extern "C" int rk_guide_eval(const PtArgs *a, const void *grid16, const void *w16,
+                             const float *master, const void *rays, const void *hits,
+                             const unsigned *counts, const void *insts, const void *tris,
+                             float *mix, unsigned grid, void *stream) {
+  hipLaunchKernelGGL(k_guide_eval, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
+                     mknet(grid16, w16, master), (const rray *)rays, (const rhit *)hits,
+                     counts, (const dinst *)insts, (const RTri *)tris, mix);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
+                                float *wsum, unsigned cap, void *stream) {
+  hipLaunchKernelGGL(k_guide_targets, dim3(256), dim3(256), 0, (hipStream_t)stream,
+                     (GdRecord *)rec, counts, (const PtPath *)paths, wsum, cap);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_train(const PtArgs *a, const void *grid16, const void *w16,
+                              const float *master, const void *rec, const unsigned *counts,
+                              unsigned cap, unsigned first, unsigned chunk, const float *wsum,
+                              float *grad, float *loss, unsigned grid, void *stream) {
+  if (!chunk) return 0;
+  hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
+                     mknet(grid16, w16, master), (const GdRecord *)rec, counts, cap, first,
+                     chunk, wsum, grad, loss);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_adam(float *master, float *grad, float *m1, float *m2, void *grid16,
+                             void *w16, unsigned step, float lr_grid, float lr_mlp,
+                             void *stream) {
+  hipLaunchKernelGGL(k_guide_adam, dim3(4096), dim3(256), 0, (hipStream_t)stream, master, grad,
+                     m1, m2, (__half *)grid16, (__bf16 *)w16, step, lr_grid, lr_mlp);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_forward(const PtArgs *a, const void *grid16, const void *w16,
+                                const float *master, const float *pn, unsigned n, float *out,
+                                void *stream) {
+  if (!n) return 0;
+  hipLaunchKernelGGL(k_guide_forward, dim3((n + 63) / 64), dim3(64), 0, (hipStream_t)stream,
+                     *a, mknet(grid16, w16, master), pn, n, out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_mfma_check(const float *A, const float *B, float *D, void *stream) {
+  hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+  return (int)hipGetLastError();
+}

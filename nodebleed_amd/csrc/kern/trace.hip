@@ -92,8 +92,9 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              rhit *__restrict__ hits, uint8_t *__restrict__ occ, uint32_t n,
              uint32_t *__restrict__ ctl, uint2 *__restrict__ spill,
              unsigned long long *__restrict__ stats, uint32_t refill,
-             uint32_t max_iter) {
+             uint32_t max_iter, const uint32_t *__restrict__ n_dev) {
   __shared__ uint32_t s_tab[2048];
+  if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
   __shared__ uint2 s_stack[REND_LDS_STACK][REND_TRACE_BLOCK];
   if (!RCP_IEEE) {
     for (uint32_t i = threadIdx.x; i < 2048; i += REND_TRACE_BLOCK)
@@ -394,10 +395,10 @@ hipError_t launch_one(uint32_t grid, hipStream_t s, const uint8_t *arena,
                       uint64_t tlas, const dinst *insts, const rray *rays,
                       rhit *hits, uint8_t *occ, uint32_t n, uint32_t *ctl,
                       uint2 *spill, unsigned long long *stats, uint32_t refill,
-                      uint32_t max_iter) {
+                      uint32_t max_iter, const uint32_t *n_dev) {
   hipLaunchKernelGGL((k_trace<C, N, I>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
                      arena, tlas, insts, rays, hits, occ, n, ctl, spill, stats,
-                     refill, max_iter);
+                     refill, max_iter, n_dev);
   return hipGetLastError();
 }
 
@@ -434,12 +435,13 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
                                void *spill, unsigned long long *stats,
                                int closest, int count, int rcp_ieee,
                                unsigned int grid, unsigned int refill,
-                               unsigned int max_iter, void *stream) {
+                               unsigned int max_iter, const unsigned int *n_dev,
+                               void *stream) {
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctl, 2u);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
-  if (n == 0) return 0;
+  if (n == 0 && !n_dev) return 0;
   const uint8_t *a = (const uint8_t *)arena;
   const dinst *in = (const dinst *)insts;
   const rray *r = (const rray *)rays;
@@ -448,14 +450,14 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
   uint2 *sp = (uint2 *)spill;
   const int k = (closest ? 4 : 0) | (count ? 2 : 0) | (rcp_ieee ? 1 : 0);
   switch (k) {
-    case 0: e = launch_one<false, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 1: e = launch_one<false, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 2: e = launch_one<false, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 3: e = launch_one<false, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 4: e = launch_one<true, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 5: e = launch_one<true, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    case 6: e = launch_one<true, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
-    default: e = launch_one<true, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter); break;
+    case 0: e = launch_one<false, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 1: e = launch_one<false, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 2: e = launch_one<false, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 3: e = launch_one<false, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 4: e = launch_one<true, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 5: e = launch_one<true, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 6: e = launch_one<true, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    default: e = launch_one<true, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
   }
   return (int)e;
 }

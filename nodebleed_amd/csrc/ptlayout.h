@@ -1,0 +1,104 @@
+/*
+ * Device records of the wavefront path tracer, shared by the host C code
+ * (host/pt.c) and the kernels (kern/pt.hip, kern/guide.hip).  Plain C.
+ */
+#ifndef PTLAYOUT_H
+#define PTLAYOUT_H
+
+#include <stdint.h>
+
+#include "dlayout.h"
+
+/* One view of a batch: camera, resolution, samples per pixel, the index of
+ * its first path in the batch (prefix sum of w*h*spp), its accumulator
+ * offset in pixels and the sample index of its first sample this frame. */
+struct PtView {
+	struct CamArgs  cam;
+	uint32_t        w, h, spp, first;
+	uint32_t        acc_ofs, sample_base, pitch, pad;
+};
+
+/* Launch constants of a frame. */
+struct PtArgs {
+	uint32_t  nviews, nlights, maxdepth, rr_depth;
+	float     bg[3];
+	uint32_t  seed;
+	float     guide_mix;      /* probability of sampling the learned proposal */
+	float     gmin[3];        /* guide grid domain: scene box */
+	float     ginv[3];        /* 1 / extent */
+	uint32_t  train_stride;   /* record vertices of every n-th path (0 = off) */
+	uint32_t  train_cap;      /* training records per batch */
+};
+
+/* Path state, 64 bytes. */
+struct PtPath {
+	float     thr[3];
+	float     L[3];
+	uint32_t  pix;        /* accumulator pixel (view offset + pixel) */
+	uint32_t  key;        /* counter-based RNG key of the path */
+	uint32_t  depth;
+	float     pdf_prev;   /* pdf of the last sampled direction */
+	uint32_t  rec;        /* training record of the current vertex, or ~0 */
+	uint32_t  pad[5];
+};
+
+/* Emissive triangle in world space, 64 bytes; pick = probability of
+ * choosing it (area * luminance / total). */
+struct PtLight {
+	float     v0[3], e0[3], e1[3], n[3], E[3];
+	float     area, pick;
+	uint32_t  pad;
+};
+
+/* ---- learned path-guiding proposal ---------------------------------------
+ * Hash-grid encoding of the shading position (GD_LEVELS levels of
+ * GD_TABLE entries, GD_FEAT features each) + the shading normal + a
+ * constant 1 -> GD_IN inputs; 3 hidden ReLU layers of GD_HID; GD_OUT
+ * outputs of which the first 5 * GD_LOBES parameterise the vMF mixture
+ * (logit, mean direction xyz, log concentration per lobe). */
+#define GD_LEVELS  12
+#define GD_TABLE   (1u << 19)
+#define GD_FEAT    2
+#define GD_IN      32
+#define GD_HID     64
+#define GD_OUT     64
+#define GD_LOBES   8
+#define GD_NMIN    16.0f
+#define GD_NMAX    1024.0f
+
+/* Offsets (in floats) of the fp32 master parameters.  Weights are stored
+ * out-major: W[n][k] multiplies input k into output n. */
+#define GD_N_GRID  ((unsigned)(GD_LEVELS * GD_TABLE * GD_FEAT))
+#define GD_OFS_W1  GD_N_GRID
+#define GD_OFS_B1  (GD_OFS_W1 + GD_HID * GD_IN)
+#define GD_OFS_W2  (GD_OFS_B1 + GD_HID)
+#define GD_OFS_B2  (GD_OFS_W2 + GD_HID * GD_HID)
+#define GD_OFS_W3  (GD_OFS_B2 + GD_HID)
+#define GD_OFS_B3  (GD_OFS_W3 + GD_HID * GD_HID)
+#define GD_OFS_W4  (GD_OFS_B3 + GD_HID)
+#define GD_OFS_B4  (GD_OFS_W4 + GD_OUT * GD_HID)
+#define GD_NPARAM  (GD_OFS_B4 + GD_OUT)
+#define GD_NMLP    (GD_NPARAM - GD_N_GRID)
+
+/* Per shading point mixture: for each lobe weight, mean xyz, kappa. */
+#define GD_MIX     (5 * GD_LOBES)
+
+/* Training record written where an indirect direction is sampled. */
+struct GdRecord {
+	float     pos[3];
+	float     nrm[3];
+	float     dir[3];
+	float     pdf;        /* density the direction was sampled with */
+	float     lbefore;    /* path luminance before this vertex's continuation */
+	float     thr;        /* throughput luminance after the continuation */
+	uint32_t  path;
+	float     target;     /* incident radiance estimate, filled after the frame */
+};
+
+#define PT_CNT_CUR     0
+#define PT_CNT_NEXT    1
+#define PT_CNT_SHADOW  2
+#define PT_CNT_TRAIN   3
+#define PT_NCOUNTS     8
+
+#endif

@@ -35,6 +35,9 @@ def oracle() -> C.CDLL:
                                   C.c_int, C.c_int]
         L.or_bruteforce.restype = None
         L.or_bruteforce.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int]
+        L.or_pt64.restype = C.c_int
+        L.or_pt64.argtypes = [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, C.c_uint,
+                              C.c_uint, C.c_uint, C.c_ulonglong, C.c_int, C.c_void_p]
         L.or_rcp.restype = C.c_float
         L.or_rcp.argtypes = [C.c_float]
         for f in ("or_xmax", "or_xmin"):
@@ -73,6 +76,25 @@ def or_bruteforce(scene: Scene, rays: np.ndarray, rcp_ieee=False):
     oracle().or_bruteforce(C.c_void_p(scene.rd), ptr(rays), ptr(hits), len(rays),
                            int(rcp_ieee))
     return hits
+
+
+def or_pt64(scene: Scene, spp: int, maxdepth: int = 4, rr_depth: int = 3, seed: int = 1,
+            threads: int = 8) -> np.ndarray:
+    """FP64 reference image (h x w x 3) of the scene's camera."""
+    v = scene.view()
+    cam = scene.camera()[:17].astype(np.float64)
+    img = np.zeros((v.height, v.width, 3), np.float64)
+    rc = oracle().or_pt64(C.c_void_p(scene.rd), ptr(cam), v.width, v.height, spp, maxdepth,
+                          rr_depth, seed, threads, ptr(img))
+    assert rc == 0
+    return img
+
+
+def rel_mse(x: np.ndarray, r: np.ndarray) -> float:
+    """sum (x - r)^2 / sum r^2 over pixels and channels."""
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(((x - r) ** 2).sum() / (r ** 2).sum())
 
 
 def reference(fast: bool = False) -> C.CDLL | None:
