@@ -46,6 +46,21 @@ def algorithmic_bytes(st: TraceStats, nrays: int) -> int:
     return nrays * S_RAY + st.nodes * S_NODE + st.leaves * S_LEAF + st.insts * S_INST
 
 
+def measured_traffic():
+    """HBM bytes per diffuse-ray launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the ratio the
+    same counters show for a plain copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "trace_terrain_diffuse_pmc.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.strip().startswith("hbm_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip().split()[0])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def timed_launches(L, scene, d_rays, d_hits, n, k):
     stream = L.rend_hip_stream()
     e0, e1 = L.rend_hip_event_create(), L.rend_hip_event_create()
@@ -82,9 +97,13 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
     run(probe)
     dt = max(time.perf_counter() - t, 1e-6)
     n = int(min(len(rays), max(20000, len(probe) / dt * seconds)))
+    # whole passes over the buffer until about `seconds` of CPU work
+    passes = max(1, int(seconds / max(dt / len(probe) * n, 1e-9)))
     t = time.perf_counter()
-    run(rays[:n])
+    for _ in range(passes):
+        run(rays[:n])
     dt = time.perf_counter() - t
+    n *= passes
     if rs is not None:
         rs.close()
     cpu = "unknown"
@@ -98,7 +117,8 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
         pass
     return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads,
             "kind": kind,
-            "sample": f"{what}, first {n} of {len(rays)} diffuse rays, {threads} threads "
+            "sample": f"{what}, {n} rays ({passes} pass(es) over the first "
+                      f"{n // passes} of {len(rays)} diffuse rays), {threads} threads "
                       f"on {cpu}, {dt:.1f} s"}
 
 
@@ -110,6 +130,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--sweep", action="store_true",
+                    help="time traversal launch shapes (blocks/CU x refill) on stderr")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after timing, run one plain copy kernel of MIB MiB "
+                         "(reference for hardware byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
@@ -149,10 +174,29 @@ def main():
     check(L.rend_hip_sync(), "sync")
     log("warmup done")
 
+    if a.sweep:
+        for mult in (0, 2, 3, 4, 6, 8):
+            for refill in (4, 8, 16, 32):
+                check(L.rend_hip_set_tuning(mult, refill), "tuning")
+                ms = timed_launches(L, scene, d_diff, d_dhit, n, 5) / 5
+                msp = timed_launches(L, scene, d_prim, d_phit, n, 5) / 5
+                log(f"sweep blocks/CU {mult or 'occ'} refill {refill}: diffuse {ms:.3f} ms "
+                    f"({n / ms / 1e3:.0f} Mrays/s), primary {msp:.3f} ms")
+        check(L.rend_hip_set_tuning(0, 16), "tuning")
+
     ms_d = timed_launches(L, scene, d_diff, d_dhit, n, a.steps)
     log(f"diffuse: {ms_d / a.steps:.3f} ms/launch")
     ms_p = timed_launches(L, scene, d_prim, d_phit, n, a.steps)
     log(f"primary: {ms_p / a.steps:.3f} ms/launch")
+
+    if a.calib_copy:
+        nb_ = a.calib_copy << 20
+        src, dst = L.rend_hip_malloc(nb_), L.rend_hip_malloc(nb_)
+        check(L.rend_hip_copy(src, dst, nb_, stream), "copy")
+        check(L.rend_hip_sync(), "sync")
+        log(f"calibration copy: {nb_} bytes read, {nb_} bytes written")
+        L.rend_hip_free(src)
+        L.rend_hip_free(dst)
 
     ms_step = ms_d / a.steps
     mrays = n / (ms_step * 1e-3) / 1e6
@@ -199,7 +243,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(),
                      "algorithmic_bytes_per_launch": int(bytes_d)},
         "cpu_baseline": cpu,
     }

@@ -72,6 +72,14 @@ int          rend_hip_occluded_host(struct rdata *rd, const struct rray *rays,
  * and the number of idle lanes that triggers a wave refill (1..64). */
 int          rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill);
 
+/* Traversal kernel variant (0..3): LDS stack depth / waves per SIMD of
+ * 16/4, 12/5, 10/6 and 8/8 (deeper entries spill to HBM).  Results are
+ * identical for every variant; only speed differs.  Default 0. */
+int          rend_hip_set_trace_variant(int variant);
+
+/* out: {blocks per CU, refill, variant} currently in use. */
+int          rend_hip_get_tuning(unsigned int out[3]);
+
 /* Builder hazards (bit set) met while building rd's trees; nonzero means the
  * reference's own build would be undefined or degenerate for this scene. */
 unsigned int rend_hip_hazards(struct rdata *rd);
@@ -111,12 +119,13 @@ int          rend_hip_render_views(struct rdata *rd, const struct rend_view *vie
                                    float *d_acc, void *stream,
                                    unsigned long long *rays);
 
-/* Times candidate launch shapes (paths in flight per batch, traversal
- * refill threshold) on one call of rend_hip_render_views with the given
- * views and keeps the fastest for later calls.  out (may be NULL) receives
- * {batch, refill, ms of the best, ms of the default}. */
+/* Times candidate launch shapes (traversal variant, then paths in flight
+ * per batch x traversal refill threshold) on one call of
+ * rend_hip_render_views with the given views and keeps the fastest for
+ * later calls.  out (may be NULL) receives {batch, refill, ms of the best,
+ * ms of the default, variant}. */
 int          rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
-                               unsigned int nviews, float *d_acc, float out[4]);
+                               unsigned int nviews, float *d_acc, float out[5]);
 
 /* HIP-graph capture of path-tracing frames (default on; guided frames are
  * always launched directly). */
@@ -180,6 +189,11 @@ int          rend_hip_free(void *d);
 int          rend_hip_memcpy_htod(void *d, const void *h, unsigned long long bytes);
 int          rend_hip_memcpy_dtoh(void *h, const void *d, unsigned long long bytes);
 int          rend_hip_sync(void);
+
+/* Streaming device copy (bytes rounded down to 16) by a plain copy kernel;
+ * used to calibrate hardware byte counters. */
+int          rend_hip_copy(const void *d_src, void *d_dst, unsigned long long bytes,
+                           void *stream);
 
 /* Event timing on a stream: returns milliseconds between two marks. */
 void        *rend_hip_event_create(void);

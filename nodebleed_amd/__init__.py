@@ -68,6 +68,8 @@ _SIGS = {
     "rend_hip_devinfo": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_uint]),
     "rend_hip_set_rcp": (C.c_int, [C.c_int]),
     "rend_hip_set_tuning": (C.c_int, [C.c_uint, C.c_uint]),
+    "rend_hip_set_trace_variant": (C.c_int, [C.c_int]),
+    "rend_hip_get_tuning": (C.c_int, [C.c_void_p]),
     "rend_hip_set_mode": (C.c_int, [C.c_int]),
     "rend_hip_pt_config": (C.c_int, [C.c_uint, C.c_uint, C.c_uint]),
     "rend_hip_upload": (C.c_int, [C.c_void_p]),
@@ -93,6 +95,7 @@ _SIGS = {
     "rend_hip_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong]),
     "rend_hip_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong]),
     "rend_hip_sync": (C.c_int, []),
+    "rend_hip_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong, C.c_void_p]),
     "rend_hip_event_create": (C.c_void_p, []),
     "rend_hip_event_record": (C.c_int, [C.c_void_p, C.c_void_p]),
     "rend_hip_event_elapsed": (C.c_float, [C.c_void_p, C.c_void_p]),

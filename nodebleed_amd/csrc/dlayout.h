@@ -58,11 +58,24 @@ struct CamArgs {
 	int    dof;
 };
 
-/* Traversal stack: entries kept in LDS per lane, deeper ones spill to a
- * global buffer of (REND_STACK_MAX - REND_LDS_STACK) entries per thread. */
-#define REND_STACK_MAX   64   /* reference bound, rend.c:834 assert */
-#define REND_LDS_STACK   16
+/* Traversal stack: the first DEPTH entries per lane are kept in LDS (DEPTH
+ * depends on the kernel variant, REND_LDS_STACK_MIN..16), deeper ones spill
+ * to a global buffer of (REND_STACK_MAX - DEPTH) entries per thread. */
+#define REND_STACK_MAX      64   /* reference bound, rend.c:834 assert */
+#define REND_LDS_STACK_MIN  8
+#define RK_TRACE_VARIANTS   4
 #define REND_TRACE_BLOCK 256
+
+/* Per-launch control block of the traversal kernel: RK_NSEG ray segments,
+ * one per XCD (workgroups are dispatched to the 8 XCDs round-robin), each
+ * with its own claim counter on a separate 128-byte line; waves claim
+ * RK_CHUNK rays at a time from their XCD's segment and steal from the other
+ * segments once it is drained.  The error word follows. */
+#define RK_NSEG        8
+#define RK_SEG_STRIDE  32
+#define RK_CHUNK       256
+#define RK_CTL_ERR     (RK_NSEG * RK_SEG_STRIDE)
+#define RK_CTL_WORDS   (RK_CTL_ERR + RK_SEG_STRIDE)
 
 /* Error bits written by kernels into the device status word. */
 #define RK_ERR_STACK  1u

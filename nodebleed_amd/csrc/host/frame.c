@@ -265,6 +265,19 @@ int rend_hip_gen_diffuse(struct rdata *rd, const struct rray *d_prim,
 	return r ? rl_hipfail(r, "diffuse rays") : 0;
 }
 
+int rk_copy(const void *src, void *dst, unsigned long long bytes, void *stream);
+
+int rend_hip_copy(const void *d_src, void *d_dst, unsigned long long bytes,
+                  void *stream)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	r = rk_copy(d_src, d_dst, bytes, st);
+	return r ? rl_hipfail(r, "copy") : 0;
+}
+
 int rend_hip_download_acc(struct rdata *rd)
 {
 	struct rl_scene *s = rl_scene(rd);

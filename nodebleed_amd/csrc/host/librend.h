@@ -10,7 +10,7 @@
 #include "rend.h"
 #include "../dlayout.h"
 
-#define RL_CTL_SLOTS  64  /* ring of (work counter, error word) pairs */
+#define RL_CTL_SLOTS  64  /* ring of traversal control blocks (RK_CTL_WORDS) */
 
 struct rl_global {
 	int                 ready;
@@ -21,7 +21,8 @@ struct rl_global {
 	unsigned            refill;      /* idle lanes that trigger a wave refill */
 	unsigned            max_iter;    /* traversal step limit per ray */
 	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
-	unsigned           *ctl;         /* RL_CTL_SLOTS x {counter, error} */
+	int                 variant;     /* traversal occupancy variant */
+	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS */
 	unsigned            ctl_next;
 	unsigned            last_slot;
 	unsigned long long *stats;       /* 4 counters of the counting variant */
@@ -104,12 +105,14 @@ int              rl_acc_ready(struct rl_scene *s, hipStream_t st);
 void             rl_camargs(const struct rcam *c, struct CamArgs *a);
 
 /* Kernel C-ABI (kern/ HIP sources). */
-int rk_trace_occupancy(int closest, int count, int rcp_ieee);
+int rk_trace_variants(void);
+int rk_trace_lds_depth(int variant);
+int rk_trace_occupancy(int closest, int count, int rcp_ieee, int variant);
 int rk_trace_launch(const void *arena, unsigned long long tlas,
                     const void *insts, const void *rays, void *hits,
                     void *occ, unsigned int n, unsigned int *ctl,
                     void *spill, unsigned long long *stats, int closest,
-                    int count, int rcp_ieee, unsigned int grid,
+                    int count, int rcp_ieee, int variant, unsigned int grid,
                     unsigned int refill, unsigned int max_iter,
                     const unsigned int *n_dev, void *stream);
 

@@ -292,7 +292,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 		for (unsigned d = 0; d < a->maxdepth; d++) {
 			unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->rays[cur], t->hits,
-			  NULL, n, G.ctl + 2 * slot, G.spill, NULL, 1, 0, G.rcp_ieee, cgrid,
+			  NULL, n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 1, 0, G.rcp_ieee, G.variant, cgrid,
 			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], st);
 			if (r)
 				return rl_hipfail(r, "extend trace");
@@ -305,7 +305,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 				return rl_hipfail(r, "shade");
 			slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->srays, NULL, t->occ,
-			  n, G.ctl + 2 * slot, G.spill, NULL, 0, 0, G.rcp_ieee, agrid, G.refill,
+			  n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 0, 0, G.rcp_ieee, G.variant, agrid, G.refill,
 			  G.max_iter, &t->counts[PT_CNT_SHADOW], st);
 			if (r)
 				return rl_hipfail(r, "shadow trace");
@@ -341,10 +341,10 @@ static unsigned long long pt_key(const struct rl_scene *s, const struct PtArgs *
 		c.sample_base = 0;
 		MIX(&c, sizeof(c));
 	}
-	const void *ptrs[6] = {acc, st, s->arena, s->insts, s->pt->paths, s->pt->views};
+	const void *ptrs[7] = {acc, st, s->arena, s->insts, s->pt->paths, s->pt->views, G.spill};
 	MIX(ptrs, sizeof(ptrs));
-	unsigned long long misc[5] = {s->tlas, s->pt->batch, G.refill, (unsigned long long)G.rcp_ieee,
-	  P.shade_mult};
+	unsigned long long misc[7] = {s->tlas, s->pt->batch, G.refill, (unsigned long long)G.rcp_ieee,
+	  P.shade_mult, (unsigned long long)G.variant, G.grid_mult};
 	MIX(misc, sizeof(misc));
 #undef MIX
 	return h;
@@ -398,17 +398,16 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 		return rl_hipfail(e, "view upload");
 	struct PtArgs a;
 	pt_args(s, &a, nv, 0x68bc21ebu);
+	/* allocations (spill area) happen before the key and any capture */
+	unsigned g0;
+	if ((r = rl_trace_grid(1, 0, &g0)) || (r = rl_trace_grid(0, 0, &g0)))
+		return r;
 	unsigned long long key = pt_key(s, &a, views, nv, acc, st);
 	/* Guided frames change the network between calls and run host-scheduled
 	 * training: launched directly. */
 	if (g_use_graph && !(s->guide && s->guide->enabled)) {
 		if (!g_graph.valid || g_graph.key != key) {
 			graph_drop();
-			/* allocations happen before capture */
-			unsigned g0;
-			if ((r = rl_trace_grid(1, 0, &g0)) || (r = rl_trace_grid(0, 0, &g0))) {
-				return r;
-			}
 			hipGraph_t gr;
 			e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
 			if (e) {
@@ -535,7 +534,7 @@ int rend_hip_render_views(struct rdata *rd, const struct rend_view *views,
 }
 
 int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
-                      unsigned int nviews, float *d_acc, float out[4])
+                      unsigned int nviews, float *d_acc, float out[5])
 {
 	static const unsigned refills[] = {8, 16, 32};
 	unsigned long long total = 0;
@@ -544,49 +543,77 @@ int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
 	unsigned batches[5] = {1u << 18, 1u << 20, 1u << 21, 1u << 22, 0};
 	struct pt_params keep = P;
 	unsigned keep_refill = G.refill;
+	int keep_variant = G.variant;
 	float best = 1e30f, dflt = -1.0f;
 	unsigned bb = P.batch, br = G.refill;
+	int bv = G.variant;
 	void *e0 = rend_hip_event_create(), *e1 = rend_hip_event_create();
 	if (!e0 || !e1)
 		return rl_fail(REND_ENODEV, "events");
 	int r = 0;
+	/* one timed call (after a capturing call) per candidate */
+#define TIME_ONE(ms_) do { \
+		if ((r = rend_hip_render_views(rd, views, nviews, 0, d_acc, NULL, NULL))) \
+			break; \
+		rend_hip_event_record(e0, NULL); \
+		r = rend_hip_render_views(rd, views, nviews, 0, d_acc, NULL, NULL); \
+		rend_hip_event_record(e1, NULL); \
+		ms_ = rend_hip_event_elapsed(e0, e1); \
+	} while (0)
+	/* traversal variant at the default batch and refill */
+	P.batch = 0;
+	G.refill = 16;
+	for (int v = 0; v < rk_trace_variants() && !r; v++) {
+		float ms = -1.0f;
+		G.variant = v;
+		TIME_ONE(ms);
+		if (!r && v == 0)
+			dflt = ms;
+		if (!r && ms > 0.0f && ms < best) {
+			best = ms;
+			bv = v;
+			bb = 0;
+			br = 16;
+		}
+	}
+	/* then batch x refill with that variant */
+	G.variant = bv;
 	for (unsigned bi = 0; bi < 5 && !r; bi++) {
 		if (batches[bi] && batches[bi] >= total * 2 && bi > 0)
 			continue;
 		for (unsigned ri = 0; ri < 3 && !r; ri++) {
+			if (batches[bi] == 0 && refills[ri] == 16)
+				continue;  /* timed above */
+			float ms = -1.0f;
 			P.batch = batches[bi];
 			G.refill = refills[ri];
-			/* first call captures, second is timed */
-			if ((r = rend_hip_render_views(rd, views, nviews, 0, d_acc, NULL, NULL)))
-				break;
-			rend_hip_event_record(e0, NULL);
-			r = rend_hip_render_views(rd, views, nviews, 0, d_acc, NULL, NULL);
-			rend_hip_event_record(e1, NULL);
-			float ms = rend_hip_event_elapsed(e0, e1);
+			TIME_ONE(ms);
 			if (!r && ms > 0.0f && ms < best) {
 				best = ms;
 				bb = P.batch;
 				br = G.refill;
 			}
-			if (!r && batches[bi] == 0 && refills[ri] == 16)
-				dflt = ms;
 		}
 	}
+#undef TIME_ONE
 	rend_hip_event_destroy(e0);
 	rend_hip_event_destroy(e1);
 	P = keep;
 	if (r) {
 		G.refill = keep_refill;
+		G.variant = keep_variant;
 		return r;
 	}
 	P.batch = bb;
 	G.refill = br;
+	G.variant = bv;
 	graph_drop();
 	if (out) {
 		out[0] = (float)bb;
 		out[1] = (float)br;
 		out[2] = best;
 		out[3] = dflt;
+		out[4] = (float)bv;
 	}
 	return 0;
 }

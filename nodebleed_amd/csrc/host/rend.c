@@ -85,9 +85,10 @@ int rl_device(void)
 	G.refill = 16;
 	G.max_iter = 1u << 22;
 	G.grid_mult = 0;
-	if ((e = hipMalloc((void **)&G.ctl, RL_CTL_SLOTS * 2 * sizeof(unsigned)))
+	G.variant = 0;
+	if ((e = hipMalloc((void **)&G.ctl, RL_CTL_SLOTS * RK_CTL_WORDS * sizeof(unsigned)))
 	      != hipSuccess ||
-	    (e = hipMemset(G.ctl, 0, RL_CTL_SLOTS * 2 * sizeof(unsigned)))
+	    (e = hipMemset(G.ctl, 0, RL_CTL_SLOTS * RK_CTL_WORDS * sizeof(unsigned)))
 	      != hipSuccess ||
 	    (e = hipMalloc((void **)&G.stats, 4 * sizeof(unsigned long long)))
 	      != hipSuccess) {
@@ -128,16 +129,32 @@ int rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill)
 	return 0;
 }
 
+int rend_hip_set_trace_variant(int variant)
+{
+	if (variant < 0 || variant >= rk_trace_variants())
+		return rl_fail(REND_EINVAL, "trace variant %d", variant);
+	G.variant = variant;
+	return 0;
+}
+
+int rend_hip_get_tuning(unsigned int out[3])
+{
+	out[0] = G.grid_mult;
+	out[1] = G.refill;
+	out[2] = (unsigned)G.variant;
+	return 0;
+}
+
 /* Persistent grid for a trace variant and the matching spill area. */
 int rl_trace_grid(int closest, int count, unsigned *grid)
 {
-	int occ = rk_trace_occupancy(closest, count, G.rcp_ieee);
+	int occ = rk_trace_occupancy(closest, count, G.rcp_ieee, G.variant);
 	if (occ < 1)
 		return rl_fail(REND_ENODEV, "trace kernel occupancy query failed");
 	unsigned mult = G.grid_mult ? G.grid_mult : (unsigned)occ;
 	unsigned g = (unsigned)G.cus * mult;
 	size_t need = (size_t)g * REND_TRACE_BLOCK *
-	  (REND_STACK_MAX - REND_LDS_STACK) * 8u;
+	  (REND_STACK_MAX - REND_LDS_STACK_MIN) * 8u;
 	if (need > G.spill_bytes) {
 		if (G.spill)
 			hipFree(G.spill);
@@ -677,8 +694,8 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 		return r;
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
-	  d_occ, n, G.ctl + 2 * slot, G.spill, count ? G.stats : NULL,
-	  closest, count, G.rcp_ieee, grid, G.refill, G.max_iter, NULL, st);
+	  d_occ, n, G.ctl + RK_CTL_WORDS * slot, G.spill, count ? G.stats : NULL,
+	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, st);
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;
@@ -707,11 +724,11 @@ int rl_check_errors(hipStream_t st)
 	if (e != hipSuccess)
 		return rl_hipfail(e, "trace");
 	unsigned w[2];
-	e = hipMemcpy(w, G.ctl + 2 * G.last_slot, sizeof(w), hipMemcpyDeviceToHost);
+	e = hipMemcpy(w, G.ctl + RK_CTL_WORDS * G.last_slot + RK_CTL_ERR - 1, sizeof(w), hipMemcpyDeviceToHost);
 	if (e != hipSuccess)
 		return rl_hipfail(e, "trace status");
 	if (w[1]) {
-		hipMemset(G.ctl + 2 * G.last_slot + 1, 0, sizeof(unsigned));
+		hipMemset(G.ctl + RK_CTL_WORDS * G.last_slot + RK_CTL_ERR, 0, sizeof(unsigned));
 		return rl_fail(REND_EOVERFLOW, "traversal %s",
 		  (w[1] & RK_ERR_STACK) ? "stack overflow (64 entries)" :
 		  "iteration limit");

@@ -201,6 +201,14 @@ __global__ void k_diffuse_rays(const rray *__restrict__ prim,
   po[1] = make_float4(o.dx, o.dy, o.dz, o.pad);
 }
 
+// Plain streaming copy (16 bytes per lane per step): the reference point
+// for converting hardware fetch/write counters into HBM bytes.
+__global__ void k_copy_f4(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_f32(float *p, size_t n, float v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -239,6 +247,14 @@ extern "C" int rk_diffuse_rays(const void *prim, const void *hits,
   hipLaunchKernelGGL(k_diffuse_rays, dim3((n + 255) / 256), dim3(256), 0,
                      (hipStream_t)stream, (const rray *)prim, (const rhit *)hits,
                      (const dinst *)insts, (const RTri *)tris, n, seed, (rray *)out);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_copy(const void *src, void *dst, unsigned long long bytes, void *stream) {
+  const size_t n = bytes / 16;
+  if (!n) return 0;
+  hipLaunchKernelGGL(k_copy_f4, dim3(4096), dim3(256), 0, (hipStream_t)stream,
+                     (const float4 *)src, (float4 *)dst, n);
   return (int)hipGetLastError();
 }
 

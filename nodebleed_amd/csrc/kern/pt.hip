@@ -372,6 +372,9 @@ __global__ void k_pt_shadow(PtPath *__restrict__ paths, const uint32_t *__restri
 // on the host side) and clears the shadow queue.
 __global__ void k_pt_advance(uint32_t *counts) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    // rays traced this segment: extension queue + shadow queue
+    *(unsigned long long *)&counts[4] += (unsigned long long)counts[PT_CNT_CUR] +
+                                         counts[PT_CNT_SHADOW];
     counts[PT_CNT_CUR] = counts[PT_CNT_NEXT];
     counts[PT_CNT_NEXT] = 0;
     counts[PT_CNT_SHADOW] = 0;

@@ -15,11 +15,14 @@
 //   * among hits in a leaf, the smallest t and, on ties, the highest lane.
 // The file is compiled with -ffp-contract=off; fused operations are explicit.
 //
-// Execution model: one ray per lane, wave64, persistent workgroups.  Each
-// wave keeps 64 lanes busy by refilling idle lanes from a global ray counter
-// with one atomic per refill (ballot + mbcnt prefix gives each idle lane its
-// ray).  The stack's first REND_LDS_STACK entries live in LDS, deeper entries
-// in a per-thread global spill area.
+// Execution model: one ray per lane, wave64, persistent workgroups.  The rays
+// are split into RK_NSEG segments, one per XCD; a wave claims RK_CHUNK rays
+// at a time from its own XCD's segment with one atomic (and steals from the
+// other segments once its own is drained), then hands claimed rays to idle
+// lanes whenever at least `refill` lanes are idle (ballot + mbcnt prefix gives
+// each idle lane its ray).  The stack's first DEPTH entries live in LDS,
+// deeper entries in a per-thread global spill area; DEPTH and the waves per
+// SIMD the register budget must allow are compile-time variants.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -58,52 +61,60 @@ __device__ __forceinline__ void ray_setup(RayS &r, float ox, float oy, float oz,
 
 // x86 RCPSS from the captured mantissa table (rule proven exhaustively by
 // gen_rcp_table.c).
-__device__ __forceinline__ float rcp_table(float x, const uint32_t *tab) {
+// Table entries keep 12 significant bits (the low 11 mantissa bits of every
+// captured result are zero), so LDS holds them as 16-bit values.
+__device__ __forceinline__ float rcp_table(float x, const uint16_t *tab) {
   uint32_t u = __float_as_uint(x);
   uint32_t s = u & 0x80000000u;
   uint32_t e = (u >> 23) & 0xffu;
   uint32_t m = u & 0x7fffffu;
-  uint32_t r = s | ((253u - e) << 23) | tab[m >> 12];
+  uint32_t r = s | ((253u - e) << 23) | ((uint32_t)tab[m >> 12] << 11);
   r = (e >= 253u) ? s : r;
   r = (e == 255u && m != 0u) ? (u | 0x00400000u) : r;
   r = (e == 0u) ? (s | 0x7f800000u) : r;
   return __uint_as_float(r);
 }
 
+// Stack entries [0, DEPTH) in LDS, deeper ones in the per-thread spill area
+// (interleaved across threads so equal depths coalesce).
+template <int DEPTH>
 struct Stack {
   uint2 (*lds)[REND_TRACE_BLOCK];
   uint2 *spill;
   uint32_t tid, gtid, nthreads;
   __device__ __forceinline__ void put(uint32_t i, uint32_t ofs, float d) {
     uint2 e = make_uint2(ofs, __float_as_uint(d));
-    if (i < REND_LDS_STACK) lds[i][tid] = e;
-    else spill[(size_t)(i - REND_LDS_STACK) * nthreads + gtid] = e;
+    if (i < DEPTH) lds[i][tid] = e;
+    else spill[(size_t)(i - DEPTH) * nthreads + gtid] = e;
   }
   __device__ __forceinline__ uint2 get(uint32_t i) const {
-    if (i < REND_LDS_STACK) return lds[i][tid];
-    return spill[(size_t)(i - REND_LDS_STACK) * nthreads + gtid];
+    if (i < DEPTH) return lds[i][tid];
+    return spill[(size_t)(i - DEPTH) * nthreads + gtid];
   }
 };
 
-template <bool CLOSEST, bool COUNT, bool RCP_IEEE>
-__global__ __launch_bounds__(REND_TRACE_BLOCK)
+// DEPTH: LDS stack entries per lane; MINW: minimum waves per SIMD the
+// register allocation must allow (occupancy variants; LDS per block is
+// 4 KB table + DEPTH * 2 KB).
+template <bool CLOSEST, bool COUNT, bool RCP_IEEE, int DEPTH, int MINW>
+__global__ __launch_bounds__(REND_TRACE_BLOCK, MINW)
 void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              const dinst *__restrict__ insts, const rray *__restrict__ rays,
              rhit *__restrict__ hits, uint8_t *__restrict__ occ, uint32_t n,
              uint32_t *__restrict__ ctl, uint2 *__restrict__ spill,
              unsigned long long *__restrict__ stats, uint32_t refill,
              uint32_t max_iter, const uint32_t *__restrict__ n_dev) {
-  __shared__ uint32_t s_tab[2048];
+  __shared__ uint16_t s_tab[2048];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
-  __shared__ uint2 s_stack[REND_LDS_STACK][REND_TRACE_BLOCK];
+  __shared__ uint2 s_stack[DEPTH][REND_TRACE_BLOCK];
   if (!RCP_IEEE) {
     for (uint32_t i = threadIdx.x; i < 2048; i += REND_TRACE_BLOCK)
-      s_tab[i] = c_rcp_tab[i];
+      s_tab[i] = (uint16_t)(c_rcp_tab[i] >> 11);
     __syncthreads();
   }
 
   const uint32_t lane = threadIdx.x & 63u;
-  Stack st;
+  Stack<DEPTH> st;
   st.lds = s_stack;
   st.spill = spill;
   st.tid = threadIdx.x;
@@ -112,6 +123,9 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
 
   bool active = false;
   bool exhausted = false;  // wave-uniform
+  // wave-uniform claim state: rays [lnext, lend) are claimed but unassigned
+  uint32_t lnext = 0, lend = 0, tried = 0;
+  const uint32_t myseg = blockIdx.x % RK_NSEG;
   uint32_t rid = 0;
   RayS wr, cr;
   float t = 0.0f, tu = 0.0f, tv = 0.0f;
@@ -123,19 +137,32 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   uint64_t c_nodes = 0, c_leaves = 0, c_insts = 0, c_rays = 0;
 
   for (;;) {
-    // ---- refill idle lanes from the global ray counter ------------------
+    // ---- refill idle lanes from the wave's claimed chunk ------------------
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle >= refill) {
-      uint32_t first = 0;
-      if (lane == 0) first = atomicAdd(&ctl[0], nidle);
-      first = __shfl(first, 0);
-      if (first + nidle >= n) exhausted = true;
+      // claim a new chunk when the current one is used up (one atomic per
+      // RK_CHUNK rays), first from this XCD's segment, then from the others
+      while (lnext == lend && !exhausted) {
+        const uint32_t seg = (myseg + tried) % RK_NSEG;
+        const uint32_t lo = (uint32_t)(((uint64_t)n * seg) / RK_NSEG);
+        const uint32_t len = (uint32_t)(((uint64_t)n * (seg + 1)) / RK_NSEG) - lo;
+        uint32_t got = 0;
+        if (lane == 0) got = atomicAdd(&ctl[seg * RK_SEG_STRIDE], RK_CHUNK);
+        got = __shfl(got, 0);
+        if (got < len) {
+          lnext = lo + got;
+          lend = lo + min(got + RK_CHUNK, len);
+        } else if (++tried == RK_NSEG) {
+          exhausted = true;
+        }
+      }
+      const uint32_t take = min(nidle, lend - lnext);
       if (!active) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(
             (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-        const uint32_t r = first + below;
-        if (r < n) {
+        const uint32_t r = lnext + below;
+        if (below < take) {
           rid = r;
           const float4 a = *(const float4 *)&rays[r].ox;
           const float4 b = *(const float4 *)&rays[r].dx;
@@ -152,6 +179,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           if (COUNT) c_rays++;
         }
       }
+      lnext += take;
     }
     if (__ballot(active) == 0) {
       if (exhausted) break;
@@ -163,7 +191,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     bool need_pop = false;
     bool done = false;
     if (++iter > max_iter) {
-      atomicOr(&ctl[1], RK_ERR_ITER);
+      atomicOr(&ctl[RK_CTL_ERR], RK_ERR_ITER);
       done = true;
     } else if (!(ofs & DNODE_LEAF)) {
       const dnode *nd = (const dnode *)(arena + base + ofs);
@@ -211,7 +239,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       } else if (sp + hitcnt > REND_STACK_MAX) {
         // The reference asserts here (rend.c:834); report instead of writing
         // past the stack.
-        atomicOr(&ctl[1], RK_ERR_STACK);
+        atomicOr(&ctl[RK_CTL_ERR], RK_ERR_STACK);
         done = true;
       } else {
         // Highest rank among hits: nearest (closest hit) or, for the any-hit
@@ -390,74 +418,134 @@ __global__ void k_zero_u32(uint32_t *p, uint32_t n) {
   if (i < n) p[i] = 0u;
 }
 
-template <bool C, bool N, bool I>
-hipError_t launch_one(uint32_t grid, hipStream_t s, const uint8_t *arena,
-                      uint64_t tlas, const dinst *insts, const rray *rays,
-                      rhit *hits, uint8_t *occ, uint32_t n, uint32_t *ctl,
-                      uint2 *spill, unsigned long long *stats, uint32_t refill,
-                      uint32_t max_iter, const uint32_t *n_dev) {
-  hipLaunchKernelGGL((k_trace<C, N, I>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
-                     arena, tlas, insts, rays, hits, occ, n, ctl, spill, stats,
-                     refill, max_iter, n_dev);
+struct LaunchArgs {
+  const uint8_t *arena;
+  uint64_t tlas;
+  const dinst *insts;
+  const rray *rays;
+  rhit *hits;
+  uint8_t *occ;
+  uint32_t n;
+  uint32_t *ctl;
+  uint2 *spill;
+  unsigned long long *stats;
+  uint32_t refill, max_iter;
+  const uint32_t *n_dev;
+};
+
+template <bool C, bool N, bool I, int D, int W>
+hipError_t launch_one(uint32_t grid, hipStream_t s, const LaunchArgs &a) {
+  hipLaunchKernelGGL((k_trace<C, N, I, D, W>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
+                     a.arena, a.tlas, a.insts, a.rays, a.hits, a.occ, a.n, a.ctl,
+                     a.spill, a.stats, a.refill, a.max_iter, a.n_dev);
   return hipGetLastError();
 }
 
-template <bool C, bool N, bool I>
+template <bool C, bool N, bool I, int D, int W>
 int occ_one() {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (k_trace<C, N, I>),
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, (k_trace<C, N, I, D, W>),
                                                     REND_TRACE_BLOCK, 0) != hipSuccess)
     return -1;
   return b;
 }
+
+// Occupancy variants (LDS stack depth, waves per SIMD); LDS per block is
+// 4 KB + depth * 2 KB, so 160 KB of LDS holds 4, 5, 6 and 8 blocks.
+// The counting kernel exists for variant 0 only.
+#define RK_VARIANTS(X) X(0, 16, 4) X(1, 12, 5) X(2, 10, 6) X(3, 8, 8)
+
+template <bool C, bool I>
+hipError_t launch_v(int v, uint32_t grid, hipStream_t s, const LaunchArgs &a) {
+#define RK_L(idx, d, w) case idx: return launch_one<C, false, I, d, w>(grid, s, a);
+  switch (v) { RK_VARIANTS(RK_L) default: break; }
+#undef RK_L
+  return hipErrorInvalidValue;
+}
+
+template <bool C, bool I>
+int occ_v(int v) {
+#define RK_O(idx, d, w) case idx: return occ_one<C, false, I, d, w>();
+  switch (v) { RK_VARIANTS(RK_O) default: break; }
+#undef RK_O
+  return -1;
+}
 }  // namespace
 
 // C-ABI between the host library and the traversal kernels.
-extern "C" int rk_trace_occupancy(int closest, int count, int rcp_ieee) {
-  const int k = (closest ? 4 : 0) | (count ? 2 : 0) | (rcp_ieee ? 1 : 0);
+extern "C" int rk_trace_variants(void) { return RK_TRACE_VARIANTS; }
+
+extern "C" int rk_trace_occupancy(int closest, int count, int rcp_ieee, int variant) {
+  if (count) {
+    const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
+    switch (k) {
+      case 0: return occ_one<false, true, false, 16, 4>();
+      case 1: return occ_one<false, true, true, 16, 4>();
+      case 2: return occ_one<true, true, false, 16, 4>();
+      default: return occ_one<true, true, true, 16, 4>();
+    }
+  }
+  const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
   switch (k) {
-    case 0: return occ_one<false, false, false>();
-    case 1: return occ_one<false, false, true>();
-    case 2: return occ_one<false, true, false>();
-    case 3: return occ_one<false, true, true>();
-    case 4: return occ_one<true, false, false>();
-    case 5: return occ_one<true, false, true>();
-    case 6: return occ_one<true, true, false>();
-    default: return occ_one<true, true, true>();
+    case 0: return occ_v<false, false>(variant);
+    case 1: return occ_v<false, true>(variant);
+    case 2: return occ_v<true, false>(variant);
+    default: return occ_v<true, true>(variant);
   }
 }
 
-// Zeroes ctl[0..nctl) with a kernel (safe inside captured graphs), then
+// LDS stack depth of a variant (the spill area holds the remaining entries).
+extern "C" int rk_trace_lds_depth(int variant) {
+#define RK_D(idx, d, w) case idx: return d;
+  switch (variant) { RK_VARIANTS(RK_D) default: break; }
+#undef RK_D
+  return -1;
+}
+
+// Zeroes the claim counters with a kernel (safe inside captured graphs), then
 // launches the persistent traversal grid.  Returns a hipError_t value.
 extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
                                const void *insts, const void *rays, void *hits,
                                void *occ, unsigned int n, unsigned int *ctl,
                                void *spill, unsigned long long *stats,
-                               int closest, int count, int rcp_ieee,
+                               int closest, int count, int rcp_ieee, int variant,
                                unsigned int grid, unsigned int refill,
                                unsigned int max_iter, const unsigned int *n_dev,
                                void *stream) {
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, ctl, 2u);
+  hipLaunchKernelGGL(k_zero_u32, dim3(4), dim3(64), 0, s, ctl, (uint32_t)RK_CTL_ERR);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return (int)e;
   if (n == 0 && !n_dev) return 0;
-  const uint8_t *a = (const uint8_t *)arena;
-  const dinst *in = (const dinst *)insts;
-  const rray *r = (const rray *)rays;
-  rhit *h = (rhit *)hits;
-  uint8_t *o = (uint8_t *)occ;
-  uint2 *sp = (uint2 *)spill;
-  const int k = (closest ? 4 : 0) | (count ? 2 : 0) | (rcp_ieee ? 1 : 0);
+  LaunchArgs a;
+  a.arena = (const uint8_t *)arena;
+  a.tlas = tlas;
+  a.insts = (const dinst *)insts;
+  a.rays = (const rray *)rays;
+  a.hits = (rhit *)hits;
+  a.occ = (uint8_t *)occ;
+  a.n = n;
+  a.ctl = ctl;
+  a.spill = (uint2 *)spill;
+  a.stats = stats;
+  a.refill = refill;
+  a.max_iter = max_iter;
+  a.n_dev = n_dev;
+  const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
+  if (count) {
+    switch (k) {
+      case 0: e = launch_one<false, true, false, 16, 4>(grid, s, a); break;
+      case 1: e = launch_one<false, true, true, 16, 4>(grid, s, a); break;
+      case 2: e = launch_one<true, true, false, 16, 4>(grid, s, a); break;
+      default: e = launch_one<true, true, true, 16, 4>(grid, s, a); break;
+    }
+    return (int)e;
+  }
   switch (k) {
-    case 0: e = launch_one<false, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 1: e = launch_one<false, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 2: e = launch_one<false, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 3: e = launch_one<false, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 4: e = launch_one<true, false, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 5: e = launch_one<true, false, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    case 6: e = launch_one<true, true, false>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
-    default: e = launch_one<true, true, true>(grid, s, a, tlas, in, r, h, o, n, ctl, sp, stats, refill, max_iter, n_dev); break;
+    case 0: e = launch_v<false, false>(variant, grid, s, a); break;
+    case 1: e = launch_v<false, true>(variant, grid, s, a); break;
+    case 2: e = launch_v<true, false>(variant, grid, s, a); break;
+    default: e = launch_v<true, true>(variant, grid, s, a); break;
   }
   return (int)e;
 }

@@ -1,0 +1,202 @@
+"""CPU (numpy) restatement of the learned path-guiding model for tests:
+hash-grid encoding, 3x64 ReLU MLP, vMF mixture, the weighted negative
+log-likelihood loss and its analytic gradient.  Constants mirror
+nodebleed_amd/csrc/ptlayout.h.  The analytic gradient is pinned by finite
+differences (tests/test_cpu_guide.py); the GPU kernels are compared with it
+(tests/test_gpu_guide.py)."""
+from __future__ import annotations
+
+import numpy as np
+
+LEVELS, TABLE, FEAT = 12, 1 << 19, 2
+GIN, HID, GOUT, LOBES = 32, 64, 64, 8
+NMIN, NMAX = 16.0, 1024.0
+LOGK_MIN, LOGK_MAX = -2.0, 7.0
+
+N_GRID = LEVELS * TABLE * FEAT
+OFS_W1 = N_GRID
+OFS_B1 = OFS_W1 + HID * GIN
+OFS_W2 = OFS_B1 + HID
+OFS_B2 = OFS_W2 + HID * HID
+OFS_W3 = OFS_B2 + HID
+OFS_B3 = OFS_W3 + HID * HID
+OFS_W4 = OFS_B3 + HID
+OFS_B4 = OFS_W4 + GOUT * HID
+NPARAM = OFS_B4 + GOUT
+
+RECORD = np.dtype([("pos", "<f4", 3), ("nrm", "<f4", 3), ("dir", "<f4", 3), ("pdf", "<f4"),
+                   ("lbefore", "<f4"), ("thr", "<f4"), ("path", "<u4"), ("target", "<f4")])
+
+
+def resolutions():
+    b = np.exp((np.log(NMAX) - np.log(NMIN)) / (LEVELS - 1))
+    res = np.floor(NMIN * b ** np.arange(LEVELS)).astype(np.float32)
+    dense = (res.astype(np.float64) + 1) ** 3 <= TABLE
+    return res, dense
+
+
+def bf16(x):
+    """Round float32 to bfloat16 (nearest even), returned as float32."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def scene_box(aabbs):
+    lo = aabbs["min"].min(axis=0).astype(np.float32)
+    hi = aabbs["max"].max(axis=0).astype(np.float32)
+    ext = hi - lo
+    pad = np.float32(0.01) * ext + np.float32(1e-4)
+    return lo - pad, np.float32(1.0) / (ext + np.float32(2.0) * pad)
+
+
+def corners(q, level):
+    """Indices (n, 8) and weights (n, 8) of the level's 8 corners."""
+    res, dense = resolutions()
+    r = res[level]
+    x = q.astype(np.float32) * r
+    f = np.floor(x)
+    fr = x - f
+    i0 = f.astype(np.int64)
+    idx = np.zeros((len(q), 8), np.int64)
+    w = np.ones((len(q), 8), np.float64)
+    for c in range(8):
+        bits = np.array([(c >> a) & 1 for a in range(3)])
+        ix = i0 + bits
+        w[:, c] = np.prod(np.where(bits, fr, 1.0 - fr), axis=1)
+        if dense[level]:
+            r1 = int(r) + 1
+            idx[:, c] = ix[:, 0] + ix[:, 1] * r1 + ix[:, 2] * r1 * r1
+        else:
+            h = (ix[:, 0].astype(np.uint64) ^ (ix[:, 1].astype(np.uint64) * 2654435761)
+                 ^ (ix[:, 2].astype(np.uint64) * 805459861)) & np.uint64(0xFFFFFFFF)
+            idx[:, c] = (h & np.uint64(TABLE - 1)).astype(np.int64)
+    return idx, w
+
+
+def normalise(pos, gmin, ginv):
+    q = (pos.astype(np.float32) - gmin) * ginv
+    return np.clip(q, 0.0, np.float32(0.99999))
+
+
+def encode(p, pos, nrm, gmin, ginv, grid_round=None):
+    grid = p[:N_GRID].reshape(LEVELS, TABLE, FEAT)
+    if grid_round is not None:
+        grid = grid_round(grid)
+    q = normalise(pos, gmin, ginv)
+    x = np.zeros((len(pos), GIN), np.float64)
+    for l in range(LEVELS):
+        idx, w = corners(q, l)
+        g = grid[l][idx]                    # n, 8, 2
+        x[:, 2 * l:2 * l + 2] = (w[..., None] * g).sum(axis=1)
+    x[:, 24:27] = nrm
+    x[:, 27] = 1.0
+    return x
+
+
+def weights(p):
+    W1 = p[OFS_W1:OFS_B1].reshape(HID, GIN)
+    W2 = p[OFS_W2:OFS_B2].reshape(HID, HID)
+    W3 = p[OFS_W3:OFS_B3].reshape(HID, HID)
+    W4 = p[OFS_W4:OFS_B4].reshape(GOUT, HID)
+    return (W1, p[OFS_B1:OFS_W2]), (W2, p[OFS_B2:OFS_W3]), (W3, p[OFS_B3:OFS_W4]), \
+        (W4, p[OFS_B4:NPARAM])
+
+
+def forward(p, x, emulate_bf16=False):
+    """Raw outputs (n, GOUT) and the hidden activations."""
+    if emulate_bf16:
+        def rnd(a):
+            return bf16(np.asarray(a, np.float32)).astype(np.float64)
+    else:
+        def rnd(a):
+            return np.asarray(a, np.float64)
+    h = rnd(x)
+    acts = [h]
+    layers = weights(p)
+    for j, (W, b) in enumerate(layers):
+        z = h @ rnd(W).T + b
+        if j < 3:
+            h = rnd(np.maximum(z, 0.0))
+            acts.append(h)
+        else:
+            h = z
+    return h, acts
+
+
+def mixture(o):
+    lg = o[:, 0:5 * LOBES:5]
+    e = np.exp(lg - lg.max(axis=1, keepdims=True))
+    pi = e / e.sum(axis=1, keepdims=True)
+    m = np.stack([o[:, 5 * j + 1:5 * j + 4] for j in range(LOBES)], axis=1)
+    ln = np.linalg.norm(m, axis=2, keepdims=True)
+    mu = np.where(ln < 1e-6, np.array([0, 0, 1.0]), m / np.maximum(ln, 1e-30))
+    lk = np.clip(o[:, 4:5 * LOBES:5], LOGK_MIN, LOGK_MAX)
+    return pi, mu, np.exp(lk)
+
+
+def vmf(k, c):
+    return k / (2 * np.pi * -np.expm1(-2 * k)) * np.exp(k * (c - 1))
+
+
+def pdf(o, d):
+    pi, mu, k = mixture(o)
+    c = np.einsum("nkj,nj->nk", mu, d)
+    return (pi * vmf(k, c)).sum(axis=1)
+
+
+def loss(p, rec, gmin, ginv):
+    """Mean over records of -(target / mean target) log p(dir)."""
+    x = encode(p, rec["pos"], rec["nrm"], gmin, ginv)
+    o, _ = forward(p, x)
+    w = rec["target"].astype(np.float64)
+    w = w / w.mean() / len(w)
+    return float(-(w * np.log(np.maximum(pdf(o, rec["dir"].astype(np.float64)), 1e-20))).sum())
+
+
+def grad(p, rec, gmin, ginv):
+    """Analytic gradient of loss() with respect to all parameters."""
+    pos, nrm = rec["pos"], rec["nrm"]
+    d = rec["dir"].astype(np.float64)
+    x = encode(p, pos, nrm, gmin, ginv)
+    o, acts = forward(p, x)
+    w = rec["target"].astype(np.float64)
+    w = w / w.mean() / len(w)
+    pi, mu, k = mixture(o)
+    c = np.einsum("nkj,nj->nk", mu, d)
+    v = vmf(k, c)
+    pp = np.maximum((pi * v).sum(axis=1), 1e-20)
+    r = pi * v / pp[:, None]
+    go = np.zeros_like(o)
+    go[:, 0:5 * LOBES:5] = -w[:, None] * (r - pi)
+    m = np.stack([o[:, 5 * j + 1:5 * j + 4] for j in range(LOBES)], axis=1)
+    ln = np.linalg.norm(m, axis=2)
+    s = -w[:, None] * r * k / np.maximum(ln, 1e-30)
+    gm = s[..., None] * (d[:, None, :] - c[..., None] * mu)
+    for j in range(LOBES):
+        go[:, 5 * j + 1:5 * j + 4] = gm[:, j]
+    lk = o[:, 4:5 * LOBES:5]
+    inside = (lk > LOGK_MIN) & (lk < LOGK_MAX)
+    dl = 1 / k - 2 / np.expm1(2 * k) + (c - 1)
+    go[:, 4:5 * LOBES:5] = np.where(inside, -w[:, None] * r * k * dl, 0.0)
+    g = np.zeros(NPARAM, np.float64)
+    layers = weights(p)
+    ofs = [(OFS_W1, OFS_B1), (OFS_W2, OFS_B2), (OFS_W3, OFS_B3), (OFS_W4, OFS_B4)]
+    dz = go
+    for j in (3, 2, 1, 0):
+        W, _ = layers[j]
+        X = acts[j]
+        g[ofs[j][0]:ofs[j][0] + W.size] += (dz.T @ X).ravel()
+        g[ofs[j][1]:ofs[j][1] + len(layers[j][1])] += dz.sum(axis=0)
+        dx = dz @ W
+        if j > 0:
+            dz = dx * (acts[j] > 0)
+        else:
+            dinput = dx
+    q = normalise(pos, gmin, ginv)
+    for l in range(LEVELS):
+        idx, wc = corners(q, l)
+        for f in range(FEAT):
+            contrib = wc * dinput[:, 2 * l + f][:, None]
+            np.add.at(g, l * TABLE * FEAT + idx.ravel() * FEAT + f, contrib.ravel())
+    return g

@@ -1,0 +1,109 @@
+"""CPU checks of the path-guiding mathematics (numpy restatement in
+guide_ref.py): vMF mixture density normalisation, the sampling routine's
+consistency with the density, the one-sample MIS weight, and the analytic
+loss gradient against finite differences."""
+import numpy as np
+
+import guide_ref as G
+
+
+def random_outputs(n, seed):
+    rng = np.random.default_rng(seed)
+    o = rng.normal(size=(n, G.GOUT))
+    o[:, 4:5 * G.LOBES:5] = rng.uniform(-1.0, 4.0, (n, G.LOBES))
+    return o
+
+
+def uniform_sphere(n, rng):
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def sample_vmf_mixture(o, u):
+    """numpy restatement of gd_sample (Jakob 2012) for checks."""
+    pi, mu, k = G.mixture(o)
+    cdf = np.cumsum(pi, axis=1)
+    j = np.minimum((u[:, :1] >= cdf).sum(axis=1), G.LOBES - 1)
+    n = np.arange(len(o))
+    m, kk = mu[n, j], k[n, j]
+    w = 1 + np.log(u[:, 1] + (1 - u[:, 1]) * np.exp(-2 * kk)) / kk
+    sw = np.sqrt(np.maximum(0, 1 - w * w))
+    phi = 2 * np.pi * u[:, 2]
+    sgn = np.where(m[:, 2] >= 0, 1.0, -1.0)
+    a = -1 / (sgn + m[:, 2])
+    b = m[:, 0] * m[:, 1] * a
+    t1 = np.stack([1 + sgn * m[:, 0] ** 2 * a, sgn * b, -sgn * m[:, 0]], 1)
+    t2 = np.stack([b, sgn + m[:, 1] ** 2 * a, -m[:, 1]], 1)
+    return w[:, None] * m + sw[:, None] * (np.cos(phi)[:, None] * t1 + np.sin(phi)[:, None] * t2)
+
+
+def test_mixture_density_integrates_to_one():
+    rng = np.random.default_rng(0)
+    o = random_outputs(4, 1)
+    d = uniform_sphere(400_000, rng)
+    for i in range(len(o)):
+        est = 4 * np.pi * G.pdf(np.repeat(o[i:i + 1], len(d), 0), d).mean()
+        assert abs(est - 1.0) < 0.03, est
+
+
+def test_sampling_matches_density():
+    """E_p[f] from samples equals the quadrature of f * p (f = smooth test
+    function), and the samples are unit vectors."""
+    rng = np.random.default_rng(2)
+    o = np.repeat(random_outputs(1, 3), 200_000, 0)
+    s = sample_vmf_mixture(o, rng.random((len(o), 3)))
+    assert np.allclose(np.linalg.norm(s, axis=1), 1.0, atol=1e-9)
+    f = lambda d: np.exp(d[:, 0]) + d[:, 1] ** 2 + 0.3 * d[:, 2]  # noqa: E731
+    mc = f(s).mean()
+    q = uniform_sphere(400_000, rng)
+    quad = 4 * np.pi * (f(q) * G.pdf(np.repeat(o[:1], len(q), 0), q)).mean()
+    assert abs(mc - quad) < 0.02 * abs(quad), (mc, quad)
+
+
+def test_one_sample_mis_weight_is_unbiased():
+    """E[(cos/pi)/p] = 1 over the hemisphere for p = a p_vmf + (1-a) cos/pi."""
+    rng = np.random.default_rng(4)
+    n = 300_000
+    o = np.repeat(random_outputs(1, 5), n, 0)
+    nrm = np.array([0.0, 0.0, 1.0])
+    alpha = 0.5
+    u = rng.random((n, 4))
+    g = sample_vmf_mixture(o, u[:, 1:])
+    r1, phi = u[:, 2], 2 * np.pi * u[:, 3]
+    c = np.stack([np.cos(phi) * np.sqrt(r1), np.sin(phi) * np.sqrt(r1), np.sqrt(1 - r1)], 1)
+    d = np.where((u[:, :1] < alpha), g, c)
+    cos = d @ nrm
+    p = alpha * G.pdf(o, d) + (1 - alpha) * np.maximum(cos, 0) / np.pi
+    w = np.where(cos > 0, (cos / np.pi) / p, 0.0)
+    assert abs(w.mean() - 1.0) < 0.01, w.mean()
+
+
+def test_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    p = np.zeros(G.NPARAM)
+    p[:G.N_GRID] = rng.uniform(-0.3, 0.3, G.N_GRID)
+    for ofs, end, fan in ((G.OFS_W1, G.OFS_B1, G.GIN), (G.OFS_W2, G.OFS_B2, G.HID),
+                          (G.OFS_W3, G.OFS_B3, G.HID), (G.OFS_W4, G.OFS_B4, G.HID)):
+        p[ofs:end] = rng.uniform(-1, 1, end - ofs) * np.sqrt(3.0 / fan)
+    p[G.OFS_B1:G.OFS_W2] = 0.1
+    p[G.OFS_B4 + 4:G.OFS_B4 + 5 * G.LOBES:5] = 1.0
+    n = 48
+    rec = np.zeros(n, G.RECORD)
+    rec["pos"] = rng.uniform(0, 1, (n, 3))
+    rec["nrm"] = uniform_sphere(n, rng)
+    rec["dir"] = uniform_sphere(n, rng)
+    rec["target"] = rng.uniform(0.1, 2.0, n)
+    gmin, ginv = np.zeros(3, np.float32), np.ones(3, np.float32) * np.float32(0.9)
+    g = G.grad(p, rec, gmin, ginv)
+    # check MLP weights, biases and grid entries that the records touch
+    q = G.normalise(rec["pos"], gmin, ginv)
+    idx, _ = G.corners(q, 5)
+    grid_ids = 5 * G.TABLE * G.FEAT + idx[:3, :2].ravel() * G.FEAT
+    ids = np.concatenate([rng.integers(G.OFS_W1, G.NPARAM, 40), grid_ids])
+    h = 1e-6
+    for i in ids:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (G.loss(pp, rec, gmin, ginv) - G.loss(pm, rec, gmin, ginv)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-5 + 2e-3 * abs(fd), (i, fd, g[i])
