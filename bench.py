@@ -122,6 +122,211 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
                       f"on {cpu}, {dt:.1f} s"}
 
 
+def emit(metric, value, unit, steps, warmup, ms, config, higher=True, dtype="f32",
+         extra=None):
+    out = {"metric": metric, "value": round(value, 3), "unit": unit, "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic (seeded procedural scenes)", "config": config}
+    if extra:
+        out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def frames(L, scene, k):
+    """k path-traced frames through rend_hip_render_frame; (seconds, rays)."""
+    rays = 0
+    samples = scene.view().samples
+    spp = scene.view().spp
+    t = time.perf_counter()
+    for _ in range(k):
+        L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
+        n = C.c_uint(0)
+        check(L.rend_hip_render_frame(scene.rd, C.byref(n)), "render")
+        rays += n.value
+        samples += spp
+    return time.perf_counter() - t, rays
+
+
+def bench_pt(a):
+    """Instanced scene: 16 meshes of 50k-500k triangles, 4,096 instances
+    (4,095 meshes + light), 1920x1080, 64 spp per frame, up to 8 segments,
+    Russian roulette from depth 3.  Step = one frame."""
+    from nodebleed_amd import REND_MODE_PATHTRACE
+    L = lib()
+    t0 = time.perf_counter()
+    scene = Scene(config=3, seed=a.seed, detail=1.0, width=W, height=H, spp=64)
+    log(f"scene: {scene.ntris} tris, {scene.view().instcnt} instances, "
+        f"build {time.perf_counter() - t0:.1f} s")
+    L.rend_staticinit(32, 8)
+    check(L.rend_hip_set_mode(REND_MODE_PATHTRACE), "mode")
+    check(L.rend_hip_pt_config(8, 3, 0), "pt config")
+    L.rend_clraccum(scene.rd)
+    frames(L, scene, a.warmup)
+    log("warmup done")
+    sec, rays = frames(L, scene, a.steps)
+    ms = sec * 1e3 / a.steps
+    log(f"pt: {ms:.1f} ms/frame, {rays / a.steps / 1e6:.1f} M rays/frame")
+    emit("path-traced Mrays/s (extension + shadow rays)", rays / sec / 1e6, "Mrays/s",
+         a.steps, a.warmup, ms,
+         {"workload": "path tracing, 16 meshes 50k-500k tris x 4,096 instances, 1920x1080, "
+                      "64 spp/frame, max depth 8, RR after depth 3",
+          "tris_unique": int(scene.ntris), "rays_per_frame": rays // a.steps,
+          "samples_per_s": round(W * H * 64 / (ms * 1e-3) / 1e6, 2)})
+    scene.close()
+    return 0
+
+
+def uneven_views(seed=1, n=32):
+    """n cameras over the terrain: resolutions from {320x240 .. 3840x2160},
+    spp uniform in 4..64, eyes on a ring above the terrain."""
+    from nodebleed_amd import VIEW
+    rng = np.random.default_rng(seed)
+    res = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    v = np.zeros(n, VIEW)
+    for i in range(n):
+        w, h = res[rng.integers(len(res))]
+        ang = rng.uniform(0, 2 * np.pi)
+        r = rng.uniform(0.3, 0.9)
+        eye = (r * np.cos(ang), rng.uniform(0.3, 0.9), 0.5 * r * np.sin(ang))
+        tgt = (rng.uniform(-0.3, 0.3), 0.0, rng.uniform(-0.15, 0.15))
+        v[i] = (eye, tgt, rng.uniform(30, 60), 1.0, 0.0, w, h, int(rng.integers(4, 65)))
+    return v
+
+
+def bench_views(a):
+    """32 uneven cameras over the 1M-triangle terrain in one batched,
+    graph-captured launch sequence; launch shape from the autotuner (run on
+    the same views at 1 spp).  Step = one call rendering every view once."""
+    L = lib()
+    scene = Scene(config=2, seed=a.seed, detail=1.0, width=64, height=64)
+    views = uneven_views(a.seed)
+    paths = int((views["w"].astype(np.int64) * views["h"] * views["spp"]).sum())
+    px = int((views["w"].astype(np.int64) * views["h"]).sum())
+    log(f"views: {len(views)} cameras, {px / 1e6:.1f} M pixels, {paths / 1e6:.0f} M paths")
+    check(L.rend_hip_pt_config(4, 3, 0), "pt config")
+    d_acc = L.rend_hip_malloc(px * 12)
+    if not d_acc:
+        raise SystemExit("accumulator allocation failed")
+    proxy = views.copy()
+    proxy["spp"] = 1
+    out = (C.c_float * 5)()
+    t = time.perf_counter()
+    check(L.rend_hip_autotune(scene.rd, proxy.ctypes.data, len(proxy), d_acc, out), "autotune")
+    log(f"autotune {time.perf_counter() - t:.1f} s: batch {int(out[0])} refill {int(out[1])} "
+        f"variant {int(out[4])}: {out[2]:.1f} ms vs default {out[3]:.1f} ms (1-spp proxy)")
+    n = C.c_ulonglong(0)
+    for i in range(a.warmup):
+        check(L.rend_hip_render_views(scene.rd, views.ctypes.data, len(views), 1000 * i, d_acc,
+                                      None, C.byref(n)), "views")
+    check(L.rend_hip_sync(), "sync")
+    log("warmup done")
+    stream = L.rend_hip_stream()
+    e0, e1 = L.rend_hip_event_create(), L.rend_hip_event_create()
+    rays = 0
+    check(L.rend_hip_event_record(e0, stream), "event")
+    for i in range(a.steps):
+        check(L.rend_hip_render_views(scene.rd, views.ctypes.data, len(views), 100000 + 100 * i,
+                                      d_acc, None, C.byref(n)), "views")
+        rays += n.value
+    check(L.rend_hip_event_record(e1, stream), "event")
+    check(L.rend_hip_sync(), "sync")
+    ms = L.rend_hip_event_elapsed(e0, e1) / a.steps
+    L.rend_hip_free(d_acc)
+    log(f"views: {ms:.1f} ms per call, {rays / a.steps / 1e6:.0f} M rays per call")
+    emit("batched path-traced Mrays/s (32 uneven views)", rays / a.steps / (ms * 1e-3) / 1e6,
+         "Mrays/s", a.steps, a.warmup, ms,
+         {"workload": "32 cameras over the 1M-tri terrain, resolutions from "
+                      "{320x240..3840x2160}, spp U{4..64}, max depth 4, RR after 3, one "
+                      "graph-captured batched launch sequence per step",
+          "pixels": px, "paths_per_step": paths, "rays_per_step": rays // a.steps,
+          "autotune": {"batch": int(out[0]), "refill": int(out[1]), "variant": int(out[4]),
+                       "proxy_ms_best": round(out[2], 2), "proxy_ms_default": round(out[3], 2)}})
+    scene.close()
+    return 0
+
+
+def bench_guide(a):
+    """Equal-time comparison on the occluded-lights terrain (8 emitters behind
+    walls), 1920x1080: relMSE of plain and guided path tracing against a
+    high-spp plain reference, as a function of wall time."""
+    from nodebleed_amd import REND_MODE_PATHTRACE
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    L = lib()
+    w, h = (W, H) if not a.guide_small else (480, 270)
+    spf = 4
+    scene = Scene(config=5, seed=a.seed, detail=1.0, width=w, height=h, spp=spf)
+    L.rend_staticinit(32, 8)
+    check(L.rend_hip_set_mode(REND_MODE_PATHTRACE), "mode")
+    maxdepth, rr = 6, 3
+    check(L.rend_hip_pt_config(maxdepth, rr, 0), "pt config")
+
+    def run(nframes, guided, sample0=0, curve_every=0, ref=None):
+        L.rend_clraccum(scene.rd)
+        if guided:
+            check(L.rend_hip_guide_enable(scene.rd, 0.5, 4, a.guide_steps, a.guide_lr, 7),
+                  "guide")
+        else:
+            L.rend_hip_guide_disable(scene.rd)
+        curve = []
+        t = time.perf_counter()
+        samples = sample0
+        for k in range(nframes):
+            L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
+            nr = C.c_uint(0)
+            check(L.rend_hip_render_frame(scene.rd, C.byref(nr)), "render")
+            samples += spf
+            if curve_every and (k + 1) % curve_every == 0:
+                el = time.perf_counter() - t
+                img = download(samples - sample0)
+                curve.append((el, (k + 1) * spf, rel_mse(img, ref) if ref is not None else None))
+                t = time.perf_counter() - el
+        el = time.perf_counter() - t
+        return el, download(samples - sample0), curve
+
+    def download(nspp):
+        check(L.rend_hip_download_acc(scene.rd), "download")
+        v = scene.view()
+        acc = scene._arr(v.acc, np.float32, w * h * 3).reshape(h, w, 3)
+        return acc.astype(np.float64) / nspp
+
+    from oracle_lib import rel_mse
+    t = time.perf_counter()
+    _, ref, _ = run(a.guide_ref_frames, False, sample0=1 << 20)
+    log(f"reference: {a.guide_ref_frames * spf} spp plain in {time.perf_counter() - t:.1f} s")
+    every = max(1, a.guide_frames // 16)
+    tp, plain, cp = run(a.guide_frames, False, curve_every=every, ref=ref)
+    err_p = rel_mse(plain, ref)
+    log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.3e}")
+    tg, guided, cg = run(a.guide_frames, True, curve_every=every, ref=ref)
+    err_g = rel_mse(guided, ref)
+    info = (C.c_float * 4)()
+    L.rend_hip_guide_info(scene.rd, info)
+    log(f"guided: {a.guide_frames * spf} spp in {tg:.2f} s, relMSE {err_g:.3e}, "
+        f"loss {info[0]:.3f}")
+    for (t1, n1, e1), (t2, n2, e2) in zip(cp, cg):
+        log(f"  spp {n1}: plain {t1:.2f} s {e1:.3e} | guided {t2:.2f} s {e2:.3e}")
+    # wall time at which guided first reaches plain's final error
+    t_eq = next((tt for tt, _, e in cg if e <= err_p), None)
+    ratio = t_eq / tp if t_eq is not None else None
+    log(f"time for guided to reach plain's final relMSE: {t_eq} s -> ratio {ratio}")
+    L.rend_hip_guide_disable(scene.rd)
+    emit("guided/plain wall-time ratio at equal relMSE", ratio if ratio else float("nan"),
+         "ratio", a.guide_frames, 0, tp * 1e3 / a.guide_frames,
+         {"workload": f"occluded-lights terrain (1M tris + 8 walls + 8 emitters), {w}x{h}, "
+                      f"{spf} spp/frame, max depth {maxdepth}, RR after {rr}; reference "
+                      f"{a.guide_ref_frames * spf} spp plain (GPU)",
+          "plain": {"seconds": round(tp, 3), "relmse": err_p,
+                    "curve": [(round(x, 3), n, e) for x, n, e in cp]},
+          "guided": {"seconds": round(tg, 3), "relmse": err_g,
+                     "curve": [(round(x, 3), n, e) for x, n, e in cg]},
+          "guide": {"mix": 0.5, "train_stride": 4, "steps_per_batch": a.guide_steps,
+                    "lr": a.guide_lr}},
+         higher=False)
+    scene.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,11 +340,22 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after timing, run one plain copy kernel of MIB MiB "
                          "(reference for hardware byte counters)")
+    ap.add_argument("--mode", default="trace", choices=("trace", "pt", "views", "guide"),
+                    help="trace: closest-hit microbench (the headline line); pt: "
+                         "instanced path tracing; views: 32 uneven cameras in one batched "
+                         "launch; guide: equal-time guided vs plain path tracing")
+    ap.add_argument("--guide-frames", type=int, default=64)
+    ap.add_argument("--guide-ref-frames", type=int, default=1024)
+    ap.add_argument("--guide-steps", type=int, default=2)
+    ap.add_argument("--guide-lr", type=float, default=1e-2)
+    ap.add_argument("--guide-small", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path", file=sys.stderr)
         sys.exit(2)
+    if a.mode != "trace":
+        sys.exit({"pt": bench_pt, "views": bench_views, "guide": bench_guide}[a.mode](a))
 
     L = lib()
     t0 = time.perf_counter()
@@ -175,14 +391,20 @@ def main():
     log("warmup done")
 
     if a.sweep:
-        for mult in (0, 2, 3, 4, 6, 8):
-            for refill in (4, 8, 16, 32):
-                check(L.rend_hip_set_tuning(mult, refill), "tuning")
-                ms = timed_launches(L, scene, d_diff, d_dhit, n, 5) / 5
-                msp = timed_launches(L, scene, d_prim, d_phit, n, 5) / 5
-                log(f"sweep blocks/CU {mult or 'occ'} refill {refill}: diffuse {ms:.3f} ms "
-                    f"({n / ms / 1e3:.0f} Mrays/s), primary {msp:.3f} ms")
-        check(L.rend_hip_set_tuning(0, 16), "tuning")
+        tun0 = (C.c_uint * 3)()
+        check(L.rend_hip_get_tuning(tun0), "tuning")
+        for variant in range(4):
+            check(L.rend_hip_set_trace_variant(variant), "variant")
+            for mult in (0, 4, 8, 12):
+                for refill in (8, 16, 32, 48):
+                    check(L.rend_hip_set_tuning(mult, refill), "tuning")
+                    ms = timed_launches(L, scene, d_diff, d_dhit, n, 5) / 5
+                    msp = timed_launches(L, scene, d_prim, d_phit, n, 5) / 5
+                    log(f"sweep variant {variant} blocks/CU {mult or 'occ'} refill {refill}: "
+                        f"diffuse {ms:.3f} ms ({n / ms / 1e3:.0f} Mrays/s), primary "
+                        f"{msp:.3f} ms")
+        check(L.rend_hip_set_trace_variant(int(tun0[2])), "variant")
+        check(L.rend_hip_set_tuning(tun0[0], tun0[1]), "tuning")
 
     ms_d = timed_launches(L, scene, d_diff, d_dhit, n, a.steps)
     log(f"diffuse: {ms_d / a.steps:.3f} ms/launch")

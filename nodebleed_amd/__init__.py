@@ -25,6 +25,10 @@ RTRI = np.dtype([("v0", "<f4", 3), ("p0", "<f4"), ("v1", "<f4", 3), ("p1", "<f4"
 RINST = np.dtype([("globinv", "<f4", 12), ("flags", "<u4"), ("triofs", "<u4"),
                   ("tricnt", "<u4"), ("pad0", "<u4")])
 AABB = np.dtype([("min", "<f4", 3), ("p0", "<f4"), ("max", "<f4", 3), ("p1", "<f4")])
+#: struct rend_view (rend_hip.h): one camera of a batched render
+VIEW = np.dtype([("eye", "<f4", 3), ("target", "<f4", 3), ("vfov", "<f4"),
+                 ("focdist", "<f4"), ("focangle", "<f4"), ("w", "<u4"), ("h", "<u4"),
+                 ("spp", "<u4")])
 
 REND_MODE_NORMALS = 0
 REND_MODE_PATHTRACE = 1
@@ -295,6 +299,42 @@ class Scene:
         acc = self._arr(self.view().acc, np.float32, w * h * 3).reshape(h, w, 3)
         self.argb = buf.reshape(h, w)
         return acc / max(samples, 1), rays
+
+    def render_views(self, views: np.ndarray, sample_base: int = 0, maxdepth: int = 4,
+                     rr_depth: int = 3, batch: int = 0, calls: int = 1):
+        """Path traces the VIEW records in one batched launch sequence per
+        call (sample_base advancing by each view's spp between calls) and
+        returns ([mean radiance h x w x 3 per view], rays traced)."""
+        L = self.L
+        check(L.rend_hip_pt_config(maxdepth, rr_depth, batch), "pt config")
+        views = np.ascontiguousarray(views, VIEW)
+        sizes = [int(v["w"]) * int(v["h"]) * 3 for v in views]
+        total = sum(sizes)
+        d_acc = L.rend_hip_malloc(total * 4)
+        if not d_acc:
+            raise RendError(L.rend_hip_lasterror(), "accumulator allocation")
+        try:
+            zero = np.zeros(total, np.float32)
+            check(L.rend_hip_memcpy_htod(d_acc, zero.ctypes.data, total * 4), "clear")
+            rays = 0
+            base = sample_base
+            for _ in range(calls):
+                n = C.c_ulonglong(0)
+                check(L.rend_hip_render_views(self.rd, views.ctypes.data, len(views), base,
+                                              d_acc, None, C.byref(n)), "render_views")
+                rays += n.value
+                base += int(views["spp"].max())
+            check(L.rend_hip_sync(), "sync")
+            acc = np.zeros(total, np.float32)
+            check(L.rend_hip_memcpy_dtoh(acc.ctypes.data, d_acc, total * 4), "download")
+        finally:
+            L.rend_hip_free(d_acc)
+        out, o = [], 0
+        for v, sz in zip(views, sizes):
+            img = acc[o:o + sz].reshape(int(v["h"]), int(v["w"]), 3)
+            out.append(img / float(int(v["spp"]) * calls))
+            o += sz
+        return out, rays
 
     def close(self):
         if self.rd:

@@ -891,7 +891,8 @@ static int config_instances(struct rs_scene *s, unsigned seed, float detail)
 	int mlit = rs_material(s, 3.0f, 3.0f, 3.0f, RFLAG_EMISSIVE);
 	float l0[3] = {-20, 140, -20}, l1[3] = {120, 140, -20};
 	float l2[3] = {120, 140, 120}, l3[3] = {-20, 140, 120};
-	if (rs_mesh_begin(s) || rs_quad(s, l0, l3, l2, l1, (unsigned)mlit))
+	/* winding l0..l3 gives the normal -y: the light faces the instances */
+	if (rs_mesh_begin(s) || rs_quad(s, l0, l1, l2, l3, (unsigned)mlit))
 		return -1;
 	int light = rs_mesh_end(s);
 	unsigned ninst = 4095;

@@ -6,6 +6,7 @@ Both images carry Monte Carlo noise; the test compares 4x4-pixel block means
 at sample counts where the noise part of relMSE is ~1e-4, so a relMSE above
 1e-3 means bias (a wrong estimator term), not noise.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -40,12 +41,143 @@ def test_box_scene_matches_fp64_oracle(maxdepth, rr):
     s.close()
 
 
-def test_furnace_box():
-    """Closed box, every surface emissive E with albedo a would give E/(1-a);
-    here: no emitters and background radiance B everywhere (open box seen
-    from outside through the open face) -> with one bounce limit the image
-    of the background equals B exactly."""
-    s = Scene(config=1, detail=0.25, width=32, height=32, spp=16)
-    img, _ = s.render(frames=1, maxdepth=1, rr_depth=0)
-    assert np.isfinite(img).all()
+def cube_scene(albedo, bg, w=48, h=48, spp=16):
+    """Flat-shaded cube [-1,1]^3 (separate vertices per face) under a uniform
+    background, no emitters."""
+    import ctypes as C
+
+    def fill(L, rs):
+        f3 = C.c_float * 3
+        m = L.rs_material(rs, *albedo, 0)
+        assert L.rs_mesh_begin(rs) == 0
+        P = [(x, y, z) for z in (-1, 1) for y in (-1, 1) for x in (-1, 1)]
+        faces = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2),
+                 (1, 3, 7, 5)]
+        for q in faces:
+            assert L.rs_quad(rs, *[f3(*P[i]) for i in q], m) == 0
+        mesh = L.rs_mesh_end(rs)
+        assert L.rs_instance(rs, mesh, f3(0, 0, 0), (C.c_float * 4)(0, 0, 0, 1),
+                             f3(1, 1, 1), 0) == 0
+        assert L.rs_camera(rs, f3(2.6, 2.1, 3.2), f3(0, 0, 0), 45.0, 1.0, 0.0) == 0
+        assert L.rs_background(rs, *bg) == 0
+
+    return Scene.from_builder(fill, width=w, height=h, spp=spp)
+
+
+def test_convex_cube_closed_form():
+    """A convex diffuse object under uniform background radiance B with no
+    emitters: every bounce off it escapes, so each sample is exactly a*B
+    (hit) or B (miss) with the Lambertian weight cos/pdf * 1/pi = 1.  Per
+    pixel v_c = B (1 - f (1 - a_c)) with one coverage fraction f for all
+    channels; interior pixels give a_c * B to float rounding."""
+    a = (0.2, 0.5, 0.8)
+    s = cube_scene(a, (1.0, 1.0, 1.0))
+    say("cube closed form: 48x48, 16 spp, depth 4")
+    img, _ = s.render(frames=1, maxdepth=4, rr_depth=0)
     s.close()
+    f = (1.0 - img) / (1.0 - np.array(a))
+    assert np.abs(f[..., 0] - f[..., 1]).max() < 1e-5
+    assert np.abs(f[..., 1] - f[..., 2]).max() < 1e-5
+    assert f.min() > -1e-5 and f.max() < 1 + 1e-5
+    inside = np.abs(f[..., 0] - 1) < 1e-5
+    outside = np.abs(f[..., 0]) < 1e-5
+    assert inside.sum() > 300 and outside.sum() > 300
+    assert np.allclose(img[inside], np.array(a), rtol=2e-6)
+
+
+def uneven_views(spp):
+    """Three cameras of different resolution and spp over the terrain."""
+    from nodebleed_amd import VIEW
+    v = np.zeros(3, VIEW)
+    v[0] = ((0, 0.95, 0.25), (0, 0, -0.05), 34, 1, 0, 96, 54, spp[0])
+    v[1] = ((0.4, 0.6, 0.6), (0, 0, 0), 45, 1, 0, 40, 40, spp[1])
+    v[2] = ((-0.5, 0.8, -0.4), (0.2, 0, 0.1), 60, 1, 0, 72, 24, spp[2])
+    return v
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    s = Scene(config=2, detail=0.1, width=64, height=64)
+    yield s
+    s.close()
+
+
+def test_batched_views_split_and_graphs_agree(terrain):
+    """One sample per pixel: each accumulator receives one addition, so the
+    result must not depend on how paths are split into wavefront batches,
+    on HIP-graph capture and replay, or on the other views of the batch."""
+    from nodebleed_amd import check, lib
+    L = lib()
+    v = uneven_views((1, 1, 1))
+    check(L.rend_hip_pt_graphs(0))
+    say("views: eager, automatic batch")
+    eager, rays = terrain.render_views(v)
+    assert rays > sum(int(x["w"] * x["h"]) for x in v)
+    say("views: eager, 2048-path batches")
+    split, _ = terrain.render_views(v, batch=2048)
+    check(L.rend_hip_pt_graphs(1))
+    try:
+        for rep in range(3):
+            say(f"views: graph call {rep}")
+            graph, _ = terrain.render_views(v)
+            check(L.rend_hip_sync())
+            for a, b in zip(eager, graph):
+                assert np.array_equal(a, b), f"graph replay {rep}"
+        for a, b in zip(eager, split):
+            assert np.array_equal(a, b), "batch split"
+        say("views: first view alone")
+        alone, _ = terrain.render_views(v[:1])
+        assert np.array_equal(alone[0], eager[0])
+    finally:
+        check(L.rend_hip_pt_graphs(1))
+    for img in eager:
+        assert np.isfinite(img).all() and (img >= 0).all() and img.mean() > 0
+
+
+def test_batched_views_converge_to_single_views(terrain):
+    """Uneven spp: batched views are unbiased estimates of the same images
+    the FP64 oracle renders for each camera alone (block means)."""
+    from nodebleed_amd import check, lib
+    v = uneven_views((4, 16, 8))
+    say("views: 3 uneven views x 24 calls")
+    imgs, _ = terrain.render_views(v, calls=24, maxdepth=3, rr_depth=0)
+    L = lib()
+    for i in (0, 2):
+        x = v[i]
+        s = Scene(config=2, detail=0.1, width=int(x["w"]), height=int(x["h"]))
+        f3 = C.c_float * 3
+        check(L.rs_set_rcam(s.rd, f3(*x["eye"]), f3(*x["target"]), float(x["vfov"]),
+                            float(x["focdist"]), float(x["focangle"]), int(x["w"]),
+                            int(x["h"])), "camera")
+        say(f"views: oracle for view {i}")
+        ref = or_pt64(s, 1024, maxdepth=3, rr_depth=0, seed=9)
+        err = rel_mse(blocks(imgs[i], 8), blocks(ref, 8))
+        say(f"view {i}: relMSE(block) {err:.2e}")
+        assert err <= 1e-3, (i, err)
+        s.close()
+
+
+def test_autotune_picks_valid_shape(terrain):
+    from nodebleed_amd import check, lib
+    L = lib()
+    v = uneven_views((2, 2, 2))
+    before, _ = terrain.render_views(v)
+    out = (C.c_float * 5)()
+    total = sum(int(x["w"] * x["h"]) * 3 for x in v)
+    d_acc = L.rend_hip_malloc(total * 4)
+    say("autotune over 3 views")
+    try:
+        check(L.rend_hip_autotune(terrain.rd, v.ctypes.data, len(v), d_acc, out), "autotune")
+    finally:
+        L.rend_hip_free(d_acc)
+    batch, refill, best, dflt, variant = list(out)
+    assert refill in (8.0, 16.0, 32.0) and 0 <= variant < 4 and 0 < best <= dflt * 1.0001
+    after, _ = terrain.render_views(v)
+    for a, b in zip(before, after):
+        np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+    tun = (C.c_uint * 3)()
+    check(L.rend_hip_get_tuning(tun))
+    assert tun[1] == int(refill) and tun[2] == int(variant)
+    check(L.rend_hip_set_trace_variant(0))
+    check(L.rend_hip_set_tuning(0, 16))
+    check(L.rend_hip_pt_config(4, 3, 0))

@@ -65,11 +65,37 @@ def test_any_hit_matches(scenes, name):
         r = rays.copy()
         # bounded shadow segments: half of the closest-hit distance or 1.0
         want_h, _ = or_trace(s, r)
-        tm = np.where(want_h["id"] != 0xFFFFFFFF, want_h["t"] * 1.5, 1.0).astype(np.float32)
+        with np.errstate(over="ignore"):
+            tm = np.where(want_h["id"] != 0xFFFFFFFF, want_h["t"] * 1.5, 1.0).astype(np.float32)
         r["tmax"] = np.minimum(r["tmax"], tm)
         want = or_occluded(s, r)
         got = s.occluded(r)
         assert np.array_equal(got, want), f"{name}/{kind}: {(got != want).sum()} differ"
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_occupancy_variants_bit_exact(scenes, variant):
+    """Every traversal variant (shallower LDS stack, more waves per SIMD,
+    deeper entries spilled to HBM) gives the oracle's hits and any-hit
+    results bit for bit."""
+    L = lib()
+    check(L.rend_hip_set_trace_variant(variant))
+    try:
+        for name, s in scenes.items():
+            for kind, rays in ray_sets(s).items():
+                say(f"variant {variant} {name}/{kind}")
+                want, _ = or_trace(s, rays)
+                got = s.trace(rays)
+                assert mismatches(got, want) == {"id": 0, "t": 0, "u": 0, "v": 0}, \
+                    f"variant {variant} {name}/{kind}"
+                r = rays.copy()
+                with np.errstate(over="ignore"):
+                    tm = np.where(want["id"] != 0xFFFFFFFF, want["t"] * 1.5, 1.0)
+                r["tmax"] = np.minimum(r["tmax"], tm.astype(np.float32))
+                assert np.array_equal(s.occluded(r), or_occluded(s, r)), \
+                    f"variant {variant} {name}/{kind} any-hit"
+    finally:
+        L.rend_hip_set_trace_variant(0)
 
 
 def test_ieee_reciprocal_mode(scenes):
