@@ -393,10 +393,10 @@ def main():
     if a.sweep:
         tun0 = (C.c_uint * 3)()
         check(L.rend_hip_get_tuning(tun0), "tuning")
-        for variant in range(4):
+        for variant in range(2):
             check(L.rend_hip_set_trace_variant(variant), "variant")
-            for mult in (0, 4, 8, 12):
-                for refill in (8, 16, 32, 48):
+            for mult in (0, 8):
+                for refill in (16, 32, 48, 56, 64):
                     check(L.rend_hip_set_tuning(mult, refill), "tuning")
                     ms = timed_launches(L, scene, d_diff, d_dhit, n, 5) / 5
                     msp = timed_launches(L, scene, d_prim, d_phit, n, 5) / 5
