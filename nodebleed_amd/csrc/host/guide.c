@@ -238,21 +238,27 @@ void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 	a->train_cap = (unsigned)(cap / g->stride + 64);
 }
 
-/* Per-batch buffers: mixtures per queued path and training records. */
+/* Per-batch buffers: mixtures per queued path and training records (as
+ * many as rl_guide_args allows for this batch size and stride). */
 int rl_guide_buffers(struct rl_scene *s, size_t cap)
 {
 	struct rl_guide *g = s->guide;
-	if (cap <= g->cap)
-		return 0;
-	dfree((void **)&g->mix);
-	dfree(&g->rec);
-	g->cap = 0;
 	size_t rc = cap / g->stride + 64;
 	hipError_t e;
-	if ((e = hipMalloc((void **)&g->mix, cap * GD_MIX * sizeof(float))) ||
-	    (e = hipMalloc(&g->rec, rc * sizeof(struct GdRecord))))
-		return rl_hipfail(e, "guide batch buffers");
-	g->cap = cap;
+	if (cap > g->cap) {
+		dfree((void **)&g->mix);
+		g->cap = 0;
+		if ((e = hipMalloc((void **)&g->mix, cap * GD_MIX * sizeof(float))))
+			return rl_hipfail(e, "guide mixture buffer");
+		g->cap = cap;
+	}
+	if (rc > g->rec_cap) {
+		dfree(&g->rec);
+		g->rec_cap = 0;
+		if ((e = hipMalloc(&g->rec, rc * sizeof(struct GdRecord))))
+			return rl_hipfail(e, "guide record buffer");
+		g->rec_cap = rc;
+	}
 	return 0;
 }
 

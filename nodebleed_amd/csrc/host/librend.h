@@ -40,7 +40,8 @@ struct rl_guide {
 	float    *mix;                      /* per queued path mixture */
 	void     *rec;                      /* training records of a batch */
 	float    *scal;                     /* [0] weight sum, [1] loss */
-	size_t    cap;
+	size_t    cap;                      /* paths the mixture buffer holds */
+	size_t    rec_cap;                  /* records the record buffer holds */
 	float     gmin[3], ginv[3];
 	float     mix_prob, lr_grid, lr_mlp;
 	unsigned  stride, steps, step, last_records;

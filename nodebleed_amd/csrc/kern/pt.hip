@@ -381,16 +381,47 @@ __global__ void k_pt_advance(uint32_t *counts) {
   }
 }
 
-// Path radiance into the accumulators; several samples of one pixel may be
-// in one batch, so additions are atomic.
+// Path radiance into the accumulators.  The samples of one pixel are
+// consecutive paths of a batch, so each wave first sums every run of equal
+// pixels (segmented reduction over the lanes: after the step with offset o a
+// lane holds the sum of its run's next 2o lanes) and only the first lane of
+// a run adds to the accumulator.  Runs can continue in the next wave or
+// batch, so the additions stay atomic.
 __global__ void k_pt_accum(const PtPath *__restrict__ paths, uint32_t n,
                            float *__restrict__ acc) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const PtPath &p = paths[i];
-  atomicAdd(&acc[3 * (size_t)p.pix + 0], p.L[0]);
-  atomicAdd(&acc[3 * (size_t)p.pix + 1], p.L[1]);
-  atomicAdd(&acc[3 * (size_t)p.pix + 2], p.L[2]);
+  const uint32_t lane = lane_id();
+  const bool valid = i < n;
+  uint32_t pix = 0xffffffffu;
+  float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
+  if (valid) {
+    const PtPath &p = paths[i];
+    pix = p.pix;
+    l0 = p.L[0];
+    l1 = p.L[1];
+    l2 = p.L[2];
+  }
+  const uint32_t prev = __shfl_up(pix, 1);
+  const bool head = valid && (lane == 0 || prev != pix);
+  // run index of each lane: heads at or below it
+  const uint64_t hm = __ballot(head);
+  const uint32_t run = __builtin_amdgcn_mbcnt_hi(
+      (uint32_t)(hm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)hm, 0u)) + (head ? 1u : 0u);
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t r2 = __shfl_down(run, o);
+    const float a0 = __shfl_down(l0, o), a1 = __shfl_down(l1, o), a2 = __shfl_down(l2, o);
+    if (lane + o < 64 && r2 == run) {
+      l0 += a0;
+      l1 += a1;
+      l2 += a2;
+    }
+  }
+  if (head) {
+    atomicAdd(&acc[3 * (size_t)pix + 0], l0);
+    atomicAdd(&acc[3 * (size_t)pix + 1], l1);
+    atomicAdd(&acc[3 * (size_t)pix + 2], l2);
+  }
 }
 
 __device__ __forceinline__ uint32_t pack8(float a) {

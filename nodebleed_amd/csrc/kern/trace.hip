@@ -41,6 +41,9 @@ struct RayS {
   float ox, oy, oz, dx, dy, dz;
   float idx, idy, idz, rx, ry, rz;
   uint32_t oct;
+  // 1/d and o/d all finite: slab distances are then never NaN, so IEEE
+  // max/min give the same comparisons as the x86 operand rules
+  bool fin;
 };
 
 __device__ __forceinline__ float xmax(float a, float b) { return a > b ? a : b; }
@@ -57,6 +60,8 @@ __device__ __forceinline__ void ray_setup(RayS &r, float ox, float oy, float oz,
   r.ry = oy * r.idy;
   r.rz = oz * r.idz;
   r.oct = (dz >= 0.0f ? 4u : 0u) | (dy >= 0.0f ? 2u : 0u) | (dx >= 0.0f ? 1u : 0u);
+  r.fin = __builtin_isfinite(r.idx) & __builtin_isfinite(r.idy) & __builtin_isfinite(r.idz) &
+          __builtin_isfinite(r.rx) & __builtin_isfinite(r.ry) & __builtin_isfinite(r.rz);
 }
 
 // x86 RCPSS from the captured mantissa table (rule proven exhaustively by
@@ -75,23 +80,61 @@ __device__ __forceinline__ float rcp_table(float x, const uint16_t *tab) {
   return __uint_as_float(r);
 }
 
-// Stack entries [0, DEPTH) in LDS, deeper ones in the per-thread spill area
-// (interleaved across threads so equal depths coalesce).
-template <int DEPTH>
-struct Stack {
-  uint2 (*lds)[REND_TRACE_BLOCK];
-  uint2 *spill;
-  uint32_t tid, gtid, nthreads;
-  __device__ __forceinline__ void put(uint32_t i, uint32_t ofs, float d) {
-    uint2 e = make_uint2(ofs, __float_as_uint(d));
-    if (i < DEPTH) lds[i][tid] = e;
-    else spill[(size_t)(i - DEPTH) * nthreads + gtid] = e;
+// Loads a node and runs the slab test of its 8 children: entry distances
+// into tmn, child words into ch, the octant's rank word into invw; returns
+// the hit mask indexed by rank.  FAST: every slab distance is known to be
+// finite (no NaN), so IEEE max/min (v_max3/v_min3) make the same decisions
+// as the reference's x86 MAXPS/MINPS operand rules.
+template <bool FAST>
+__device__ __forceinline__ uint32_t slab8(const dnode *ndp, const RayS &cr, float t,
+                                          float tmn[8], uint32_t ch[8], uint32_t &invw) {
+  // nodes sit at 256-byte aligned arena offsets
+  const dnode *nd = (const dnode *)__builtin_assume_aligned(ndp, 256);
+  // near/far bound rows of each axis (32-byte rows: min, max per axis)
+  const char *nb = (const char *)nd;
+  const uint32_t ax = (cr.oct & 1u) ? 0u : 32u, ay = (cr.oct & 2u) ? 64u : 96u,
+                 az = (cr.oct & 4u) ? 128u : 160u;
+  const float *nx = (const float *)__builtin_assume_aligned(nb + ax, 32);
+  const float *fx = (const float *)__builtin_assume_aligned(nb + (ax ^ 32u), 32);
+  const float *ny = (const float *)__builtin_assume_aligned(nb + ay, 32);
+  const float *fy = (const float *)__builtin_assume_aligned(nb + (ay ^ 32u), 32);
+  const float *nz = (const float *)__builtin_assume_aligned(nb + az, 32);
+  const float *fz = (const float *)__builtin_assume_aligned(nb + (az ^ 32u), 32);
+  float nxv[8], fxv[8], nyv[8], fyv[8], nzv[8], fzv[8];
+#pragma unroll
+  for (int k = 0; k < 8; k += 4) {
+    *(float4 *)&nxv[k] = *(const float4 *)&nx[k];
+    *(float4 *)&fxv[k] = *(const float4 *)&fx[k];
+    *(float4 *)&nyv[k] = *(const float4 *)&ny[k];
+    *(float4 *)&fyv[k] = *(const float4 *)&fy[k];
+    *(float4 *)&nzv[k] = *(const float4 *)&nz[k];
+    *(float4 *)&fzv[k] = *(const float4 *)&fz[k];
+    *(uint4 *)&ch[k] = *(const uint4 *)&nd->child[k];
   }
-  __device__ __forceinline__ uint2 get(uint32_t i) const {
-    if (i < DEPTH) return lds[i][tid];
-    return spill[(size_t)(i - DEPTH) * nthreads + gtid];
+  invw = nd->inv[cr.oct];
+  uint32_t hmr = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const float t0x = __builtin_fmaf(nxv[c], cr.idx, -cr.rx);
+    const float t0y = __builtin_fmaf(nyv[c], cr.idy, -cr.ry);
+    const float t0z = __builtin_fmaf(nzv[c], cr.idz, -cr.rz);
+    const float t1x = __builtin_fmaf(fxv[c], cr.idx, -cr.rx);
+    const float t1y = __builtin_fmaf(fyv[c], cr.idy, -cr.ry);
+    const float t1z = __builtin_fmaf(fzv[c], cr.idz, -cr.rz);
+    float lo, hi;
+    if (FAST) {
+      lo = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf(t0x, t0y), t0z), 0.0f);
+      hi = __builtin_fminf(__builtin_fminf(__builtin_fminf(t1x, t1y), t1z), t);
+    } else {
+      lo = xmax(xmax(xmax(t0x, t0y), t0z), 0.0f);
+      hi = xmin(xmin(xmin(t1x, t1y), t1z), t);
+    }
+    tmn[c] = lo;
+    const uint32_t rank = (invw >> (3 * c)) & 7u;
+    hmr |= (lo <= hi ? 1u : 0u) << rank;
   }
-};
+  return hmr;
+}
 
 // DEPTH: LDS stack entries per lane; MINW: minimum waves per SIMD the
 // register allocation must allow (occupancy variants; LDS per block is
@@ -106,7 +149,9 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              uint32_t max_iter, const uint32_t *__restrict__ n_dev) {
   __shared__ uint16_t s_tab[2048];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
-  __shared__ uint2 s_stack[DEPTH][REND_TRACE_BLOCK];
+  // stack rows [0, DEPTH) per lane, plus one junk row that branch-free
+  // pushes write for children they do not keep
+  __shared__ uint2 s_stack[DEPTH + 1][REND_TRACE_BLOCK];
   if (!RCP_IEEE) {
     for (uint32_t i = threadIdx.x; i < 2048; i += REND_TRACE_BLOCK)
       s_tab[i] = (uint16_t)(c_rcp_tab[i] >> 11);
@@ -114,12 +159,18 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   }
 
   const uint32_t lane = threadIdx.x & 63u;
-  Stack<DEPTH> st;
-  st.lds = s_stack;
-  st.spill = spill;
-  st.tid = threadIdx.x;
-  st.gtid = blockIdx.x * REND_TRACE_BLOCK + threadIdx.x;
-  st.nthreads = gridDim.x * REND_TRACE_BLOCK;
+  const uint32_t tid = threadIdx.x;
+  // Stack entries [0, DEPTH) live in LDS, deeper ones in the per-thread
+  // spill area (interleaved across threads so equal depths coalesce).
+  uint2 *const myspill = spill + blockIdx.x * REND_TRACE_BLOCK + threadIdx.x;
+  const size_t sstride = (size_t)gridDim.x * REND_TRACE_BLOCK;
+  // LDS is always addressed through a clamped row (deep entries touch the
+  // junk row) so LDS and spill accesses never share a generic pointer
+  auto put = [&](uint32_t i, uint32_t o, float d) {
+    const uint2 e = make_uint2(o, __float_as_uint(d));
+    s_stack[i < (uint32_t)DEPTH ? i : (uint32_t)DEPTH][tid] = e;
+    if (i >= (uint32_t)DEPTH) myspill[(i - DEPTH) * sstride] = e;
+  };
 
   bool active = false;
   bool exhausted = false;  // wave-uniform
@@ -127,7 +178,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   uint32_t lnext = 0, lend = 0, tried = 0;
   const uint32_t myseg = blockIdx.x % RK_NSEG;
   uint32_t rid = 0;
-  RayS wr, cr;
+  RayS cr;                    // ray of the current tree (world or instance)
+  float wo[3] = {0, 0, 0}, wd[3] = {0, 0, 0};  // world ray
   float t = 0.0f, tu = 0.0f, tv = 0.0f;
   uint32_t hid = 0xffffffffu;
   uint32_t ofs = 0, sp = 0, blas_sp = 0, instid = 0, iter = 0;
@@ -166,8 +218,9 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           rid = r;
           const float4 a = *(const float4 *)&rays[r].ox;
           const float4 b = *(const float4 *)&rays[r].dx;
-          ray_setup(wr, a.x, a.y, a.z, b.x, b.y, b.z);
-          cr = wr;
+          wo[0] = a.x; wo[1] = a.y; wo[2] = a.z;
+          wd[0] = b.x; wd[1] = b.y; wd[2] = b.z;
+          ray_setup(cr, a.x, a.y, a.z, b.x, b.y, b.z);
           t = a.w;
           tu = tv = 0.0f;
           hid = 0xffffffffu;
@@ -195,43 +248,13 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       done = true;
     } else if (!(ofs & DNODE_LEAF)) {
       const dnode *nd = (const dnode *)(arena + base + ofs);
-      const float *nx = (cr.oct & 1u) ? nd->minx : nd->maxx;
-      const float *fx = (cr.oct & 1u) ? nd->maxx : nd->minx;
-      const float *ny = (cr.oct & 2u) ? nd->miny : nd->maxy;
-      const float *fy = (cr.oct & 2u) ? nd->maxy : nd->miny;
-      const float *nz = (cr.oct & 4u) ? nd->minz : nd->maxz;
-      const float *fz = (cr.oct & 4u) ? nd->maxz : nd->minz;
-      float nxv[8], fxv[8], nyv[8], fyv[8], nzv[8], fzv[8];
-      uint32_t ch[8];
-#pragma unroll
-      for (int k = 0; k < 8; k += 4) {
-        *(float4 *)&nxv[k] = *(const float4 *)&nx[k];
-        *(float4 *)&fxv[k] = *(const float4 *)&fx[k];
-        *(float4 *)&nyv[k] = *(const float4 *)&ny[k];
-        *(float4 *)&fyv[k] = *(const float4 *)&fy[k];
-        *(float4 *)&nzv[k] = *(const float4 *)&nz[k];
-        *(float4 *)&fzv[k] = *(const float4 *)&fz[k];
-        *(uint4 *)&ch[k] = *(const uint4 *)&nd->child[k];
-      }
-      const uint32_t invw = nd->inv[cr.oct];
-      if (COUNT) c_nodes++;
-
       float tmn[8];
-      uint32_t hmr = 0;
-#pragma unroll
-      for (int c = 0; c < 8; c++) {
-        const float t0x = __builtin_fmaf(nxv[c], cr.idx, -cr.rx);
-        const float t0y = __builtin_fmaf(nyv[c], cr.idy, -cr.ry);
-        const float t0z = __builtin_fmaf(nzv[c], cr.idz, -cr.rz);
-        const float t1x = __builtin_fmaf(fxv[c], cr.idx, -cr.rx);
-        const float t1y = __builtin_fmaf(fyv[c], cr.idy, -cr.ry);
-        const float t1z = __builtin_fmaf(fzv[c], cr.idz, -cr.rz);
-        const float lo = xmax(xmax(xmax(t0x, t0y), t0z), 0.0f);
-        const float hi = xmin(xmin(xmin(t1x, t1y), t1z), t);
-        tmn[c] = lo;
-        const uint32_t rank = (invw >> (3 * c)) & 7u;
-        hmr |= (lo <= hi ? 1u : 0u) << rank;
-      }
+      uint32_t ch[8], invw;
+      // the path is chosen before the node is read, so each is one block
+      const uint32_t hmr = __ballot(!(cr.fin && t == t)) == 0
+                               ? slab8<true>(nd, cr, t, tmn, ch, invw)
+                               : slab8<false>(nd, cr, t, tmn, ch, invw);
+      if (COUNT) c_nodes++;
 
       const uint32_t hitcnt = __builtin_popcount(hmr);
       if (hitcnt == 0) {
@@ -246,37 +269,37 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         // pass, the reference's unordered push makes the highest child index
         // the next node; ranks then order by child index.
         uint32_t next = 0;
-        if (CLOSEST) {
-          const uint32_t top = 31u - __builtin_clz(hmr);
+        // order key of each child: its rank (closest hit) or its index
+        uint32_t keym = hmr;
+        if (!CLOSEST) {
+          keym = 0;
+#pragma unroll
+          for (int c = 0; c < 8; c++) keym |= ((hmr >> ((invw >> (3 * c)) & 7u)) & 1u) << c;
+        }
+        const uint32_t top = 31u - __builtin_clz(keym);
+        if (__ballot(sp + 7u > (uint32_t)DEPTH) == 0) {
+          // no lane can reach the spill area: branch-free pushes, children
+          // that are not kept go to the junk row
 #pragma unroll
           for (int c = 0; c < 8; c++) {
-            const uint32_t rank = (invw >> (3 * c)) & 7u;
-            if ((hmr >> rank) & 1u) {
-              if (rank == top) {
-                next = ch[c];
-              } else {
-                const uint32_t pos = __builtin_popcount(hmr & ((1u << rank) - 1u));
-                st.put(sp + pos, ch[c], tmn[c]);
-              }
-            }
+            const uint32_t key = CLOSEST ? (invw >> (3 * c)) & 7u : (uint32_t)c;
+            const bool hit = (keym >> key) & 1u;
+            const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
+            const uint32_t slot = (hit && key != top) ? sp + pos : (uint32_t)DEPTH;
+            s_stack[slot][tid] =
+                make_uint2(ch[c], CLOSEST ? __float_as_uint(tmn[c]) : 0u);
+            next = key == top ? ch[c] : next;
           }
         } else {
-          // Unordered: child index order (rend.c:1884-1886).
-          uint32_t hm = 0;
 #pragma unroll
           for (int c = 0; c < 8; c++) {
-            const uint32_t rank = (invw >> (3 * c)) & 7u;
-            hm |= ((hmr >> rank) & 1u) << c;
-          }
-          const uint32_t top = 31u - __builtin_clz(hm);
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            if ((hm >> c) & 1u) {
-              if ((uint32_t)c == top) {
+            const uint32_t key = CLOSEST ? (invw >> (3 * c)) & 7u : (uint32_t)c;
+            if ((keym >> key) & 1u) {
+              if (key == top) {
                 next = ch[c];
               } else {
-                const uint32_t pos = __builtin_popcount(hm & ((1u << c) - 1u));
-                st.put(sp + pos, ch[c], 0.0f);
+                const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
+                put(sp + pos, ch[c], CLOSEST ? tmn[c] : 0.0f);
               }
             }
           }
@@ -356,12 +379,12 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       const float4 m2 = *(const float4 *)&in->inv[8];
       const uint64_t blas = in->blas;
       if (COUNT) c_insts++;
-      const float ox = m0.x * wr.ox + m0.y * wr.oy + m0.z * wr.oz + m0.w;
-      const float oy = m1.x * wr.ox + m1.y * wr.oy + m1.z * wr.oz + m1.w;
-      const float oz = m2.x * wr.ox + m2.y * wr.oy + m2.z * wr.oz + m2.w;
-      const float dx = m0.x * wr.dx + m0.y * wr.dy + m0.z * wr.dz;
-      const float dy = m1.x * wr.dx + m1.y * wr.dy + m1.z * wr.dz;
-      const float dz = m2.x * wr.dx + m2.y * wr.dy + m2.z * wr.dz;
+      const float ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
+      const float oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
+      const float oz = m2.x * wo[0] + m2.y * wo[1] + m2.z * wo[2] + m2.w;
+      const float dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
+      const float dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
+      const float dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
       ray_setup(cr, ox, oy, oz, dx, dy, dz);
       in_blas = true;
       blas_sp = sp;
@@ -373,12 +396,19 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       for (;;) {
         if (in_blas && sp == blas_sp) {
           in_blas = false;
-          cr = wr;
+          ray_setup(cr, wo[0], wo[1], wo[2], wd[0], wd[1], wd[2]);
           base = tlas;
         }
         if (sp == 0) { done = true; break; }
         --sp;
-        const uint2 e = st.get(sp);
+        uint2 e = s_stack[sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH][tid];
+        if (sp >= (uint32_t)DEPTH) {
+          // volatile keeps the compiler from merging this load with the LDS
+          // one into a single generic (flat) load
+          const volatile uint32_t *q = (const volatile uint32_t *)&myspill[(sp - DEPTH) * sstride];
+          e.x = q[0];
+          e.y = q[1];
+        }
         if (!CLOSEST || __uint_as_float(e.y) <= t) { ofs = e.x; break; }
       }
     }

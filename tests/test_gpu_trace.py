@@ -14,7 +14,8 @@ import sys
 import numpy as np
 import pytest
 
-from nodebleed_amd import HIT, RAY, REND_RCP_HOST_TABLE, REND_RCP_IEEE, Scene, check, lib
+from nodebleed_amd import (HIT, RAY, REND_RCP_HOST_TABLE, REND_RCP_IEEE, Scene, check, lib,
+                           make_rays)
 from oracle_lib import or_occluded, or_trace
 from raygen import aimed_rays, edge_rays, mismatches, random_rays, ulp_diff
 
@@ -96,6 +97,38 @@ def test_occupancy_variants_bit_exact(scenes, variant):
                     f"variant {variant} {name}/{kind} any-hit"
     finally:
         L.rend_hip_set_trace_variant(0)
+
+
+def test_cube_cracks_match_the_reference_semantics():
+    """Camera rays through a two-triangles-per-face cube: with the host RCPSS
+    reciprocal about 1 % slip through the crack along a face diagonal and hit
+    the far face from inside.  The GPU must report the same hits, cracks
+    included, and none with the IEEE reciprocal."""
+    import scenes_closed as S
+    s = S.cube_scene((0.5, 0.5, 0.5), (1, 1, 1))
+    rng = np.random.default_rng(17)
+    eye = np.array([2.6, 2.1, 3.2])
+    d = rng.uniform(-1, 1, (200_000, 3)) - eye
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = make_rays(np.broadcast_to(eye, d.shape).astype(np.float32), d.astype(np.float32))
+    inv = 1 / d
+    tn = np.minimum((-1 - eye) * inv, (1 - eye) * inv).max(1)
+    L = lib()
+    try:
+        for mode in (REND_RCP_HOST_TABLE, REND_RCP_IEEE):
+            check(L.rend_hip_set_rcp(mode))
+            say(f"cube cracks rcp mode {mode}")
+            want, _ = or_trace(s, rays, rcp_ieee=mode == REND_RCP_IEEE)
+            got = s.trace(rays)
+            assert mismatches(got, want) == {"id": 0, "t": 0, "u": 0, "v": 0}, mode
+            far = (got["id"] != 0xFFFFFFFF) & (np.abs(got["t"] - tn) > 1e-3)
+            if mode == REND_RCP_HOST_TABLE:
+                assert 0.002 < far.mean() < 0.03, far.mean()
+            else:
+                assert far.sum() == 0
+    finally:
+        L.rend_hip_set_rcp(REND_RCP_HOST_TABLE)
+        s.close()
 
 
 def test_ieee_reciprocal_mode(scenes):
