@@ -68,8 +68,9 @@ int          rend_hip_trace_host(struct rdata *rd, const struct rray *rays,
 int          rend_hip_occluded_host(struct rdata *rd, const struct rray *rays,
                                     unsigned char *occ, unsigned int n);
 
-/* Traversal launch shape: persistent blocks per CU (0 = occupancy limit)
- * and the number of idle lanes that triggers a wave refill (1..64). */
+/* Traversal launch shape: persistent blocks per CU (0 = occupancy limit,
+ * the default) and the number of idle lanes that triggers a wave refill
+ * (1..64, default 48). */
 int          rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill);
 
 /* Traversal kernel variant (0..3): LDS stack depth / waves per SIMD of

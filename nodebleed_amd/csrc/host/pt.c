@@ -536,7 +536,7 @@ int rend_hip_render_views(struct rdata *rd, const struct rend_view *views,
 int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
                       unsigned int nviews, float *d_acc, float out[5])
 {
-	static const unsigned refills[] = {8, 16, 32};
+	static const unsigned refills[] = {16, 32, 48};
 	unsigned long long total = 0;
 	for (unsigned i = 0; i < nviews; i++)
 		total += (unsigned long long)views[i].w * views[i].h * views[i].spp;
@@ -562,7 +562,7 @@ int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
 	} while (0)
 	/* traversal variant at the default batch and refill */
 	P.batch = 0;
-	G.refill = 16;
+	G.refill = 48;
 	for (int v = 0; v < rk_trace_variants() && !r; v++) {
 		float ms = -1.0f;
 		G.variant = v;
@@ -573,7 +573,7 @@ int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
 			best = ms;
 			bv = v;
 			bb = 0;
-			br = 16;
+			br = 48;
 		}
 	}
 	/* then batch x refill with that variant */
@@ -582,7 +582,7 @@ int rend_hip_autotune(struct rdata *rd, const struct rend_view *views,
 		if (batches[bi] && batches[bi] >= total * 2 && bi > 0)
 			continue;
 		for (unsigned ri = 0; ri < 3 && !r; ri++) {
-			if (batches[bi] == 0 && refills[ri] == 16)
+			if (batches[bi] == 0 && refills[ri] == 48)
 				continue;  /* timed above */
 			float ms = -1.0f;
 			P.batch = batches[bi];

@@ -82,7 +82,7 @@ int rl_device(void)
 	}
 	G.cus = p.multiProcessorCount;
 	snprintf(G.name, sizeof(G.name), "%s", p.gcnArchName);
-	G.refill = 16;
+	G.refill = 48;
 	G.max_iter = 1u << 22;
 	G.grid_mult = 0;
 	G.variant = 0;

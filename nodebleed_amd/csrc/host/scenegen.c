@@ -822,23 +822,48 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 	      (unsigned)mter))
 		return -1;
 	if (lights) {
-		/* 8 small emitters, each hidden behind a wall that faces the
-		 * camera side; light reaches the view only indirectly. */
+		/* Occluded lights: 8 small emitters facing up, each inside a
+		 * narrow chimney open at the top, so that no terrain point sees
+		 * one; they light spots on a diffuse ceiling, and the terrain
+		 * receives light only after a bounce off those spots. */
 		int mwall = rs_material(s, 0.7f, 0.7f, 0.7f, 0);
-		int mlit = rs_material(s, 40.0f, 36.0f, 30.0f, RFLAG_EMISSIVE);
+		int mceil = rs_material(s, 0.8f, 0.8f, 0.8f, 0);
+		int mlit = rs_material(s, 150.0f, 135.0f, 112.0f, RFLAG_EMISSIVE);
+		/* a room: ceiling, back and side walls, open towards the camera */
+		const float cy = 0.5f, yb = -0.06f;
+		float c0[3] = {-1.1f, cy, -0.6f}, c1[3] = {1.1f, cy, -0.6f};
+		float c2[3] = {1.1f, cy, 0.6f}, c3[3] = {-1.1f, cy, 0.6f};
+		float b0[3] = {-1.1f, yb, -0.55f}, b1[3] = {1.1f, yb, -0.55f};
+		float b2[3] = {1.1f, cy, -0.55f}, b3[3] = {-1.1f, cy, -0.55f};
+		float l0[3] = {-1.05f, yb, 0.6f}, l1[3] = {-1.05f, yb, -0.6f};
+		float l2[3] = {-1.05f, cy, -0.6f}, l3[3] = {-1.05f, cy, 0.6f};
+		float r0[3] = {1.05f, yb, -0.6f}, r1[3] = {1.05f, yb, 0.6f};
+		float r2[3] = {1.05f, cy, 0.6f}, r3[3] = {1.05f, cy, -0.6f};
+		if (rs_quad(s, c0, c1, c2, c3, (unsigned)mceil) ||
+		    rs_quad(s, b0, b1, b2, b3, (unsigned)mwall) ||
+		    rs_quad(s, l0, l1, l2, l3, (unsigned)mwall) ||
+		    rs_quad(s, r0, r1, r2, r3, (unsigned)mwall))
+			return -1;
 		for (int i = 0; i < 8; i++) {
-			float cx = -0.8f + 0.2286f * (float)i;
-			float cz = (i & 1) ? -0.25f : 0.15f;
-			float y0 = -0.06f, y1 = 0.12f, hw = 0.07f;
-			float w0[3] = {cx - hw, y0, cz}, w1[3] = {cx + hw, y0, cz};
-			float w2[3] = {cx + hw, y1, cz}, w3[3] = {cx - hw, y1, cz};
-			if (rs_quad(s, w0, w1, w2, w3, (unsigned)mwall) ||
-			    rs_quad(s, w1, w0, w3, w2, (unsigned)mwall))
-				return -1;
-			float ly = 0.04f, lz = cz - 0.03f, lw = 0.03f;
-			float q0[3] = {cx - lw, ly - lw, lz}, q1[3] = {cx + lw, ly - lw, lz};
-			float q2[3] = {cx + lw, ly + lw, lz}, q3[3] = {cx - lw, ly + lw, lz};
-			if (rs_quad(s, q1, q0, q3, q2, (unsigned)mlit))
+			float cx = -0.6f + 0.4f * (float)(i & 3);
+			float cz = (i & 4) ? 0.2f : -0.25f;
+			const float hw = 0.025f, y0 = -0.06f, y1 = 0.40f;
+			/* four chimney walls */
+			const float ox[4] = {-hw, hw, hw, -hw}, oz[4] = {-hw, -hw, hw, hw};
+			for (int k = 0; k < 4; k++) {
+				int k2 = (k + 1) & 3;
+				float w0[3] = {cx + ox[k], y0, cz + oz[k]};
+				float w1[3] = {cx + ox[k2], y0, cz + oz[k2]};
+				float w2[3] = {cx + ox[k2], y1, cz + oz[k2]};
+				float w3[3] = {cx + ox[k], y1, cz + oz[k]};
+				if (rs_quad(s, w0, w1, w2, w3, (unsigned)mwall))
+					return -1;
+			}
+			/* emitter 0.06 below the opening, normal +y */
+			const float lw = 0.02f, ly = 0.34f;
+			float q0[3] = {cx - lw, ly, cz - lw}, q1[3] = {cx - lw, ly, cz + lw};
+			float q2[3] = {cx + lw, ly, cz + lw}, q3[3] = {cx + lw, ly, cz - lw};
+			if (rs_quad(s, q0, q1, q2, q3, (unsigned)mlit))
 				return -1;
 		}
 	}
@@ -846,11 +871,18 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 	float t[3] = {0, 0, 0}, q[4] = {0, 0, 0, 1}, sc[3] = {1, 1, 1};
 	if (mesh < 0 || rs_instance(s, (unsigned)mesh, t, q, sc, 0) < 0)
 		return -1;
+	if (lights) {
+		/* below the chimney tops, looking down at the terrain: neither
+		 * the emitters nor the lit ceiling spots are in view */
+		float eye[3] = {0.0f, 0.3f, 0.55f}, tg[3] = {0.0f, 0.0f, -0.1f};
+		rs_camera(s, eye, tg, 40.0f, 1.0f, 0.0f);
+		rs_background(s, 0.0f, 0.0f, 0.0f);
+		return 0;
+	}
 	/* Sees only terrain at 16:9 (every primary ray hits). */
 	float eye[3] = {0.0f, 0.95f, 0.25f}, tg[3] = {0.0f, 0.0f, -0.05f};
 	rs_camera(s, eye, tg, 34.0f, 1.0f, 0.0f);
-	rs_background(s, lights ? 0.0f : 0.4f, lights ? 0.0f : 0.4f,
-	  lights ? 0.0f : 0.4f);
+	rs_background(s, 0.4f, 0.4f, 0.4f);
 	return 0;
 }
 

@@ -265,24 +265,20 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         atomicOr(&ctl[RK_CTL_ERR], RK_ERR_STACK);
         done = true;
       } else {
-        // Highest rank among hits: nearest (closest hit) or, for the any-hit
-        // pass, the reference's unordered push makes the highest child index
-        // the next node; ranks then order by child index.
+        // Highest rank among hits (nearest) is the next node, the others are
+        // pushed farthest first.  The any-hit pass uses the same order: its
+        // t_far never shrinks, so the set of leaves it can reach, and with it
+        // the occlusion result, does not depend on the visiting order (the
+        // reference pushes unordered, rend.c:1884-1886).
         uint32_t next = 0;
-        // order key of each child: its rank (closest hit) or its index
-        uint32_t keym = hmr;
-        if (!CLOSEST) {
-          keym = 0;
-#pragma unroll
-          for (int c = 0; c < 8; c++) keym |= ((hmr >> ((invw >> (3 * c)) & 7u)) & 1u) << c;
-        }
+        const uint32_t keym = hmr;
         const uint32_t top = 31u - __builtin_clz(keym);
         if (__ballot(sp + 7u > (uint32_t)DEPTH) == 0) {
           // no lane can reach the spill area: branch-free pushes, children
           // that are not kept go to the junk row
 #pragma unroll
           for (int c = 0; c < 8; c++) {
-            const uint32_t key = CLOSEST ? (invw >> (3 * c)) & 7u : (uint32_t)c;
+            const uint32_t key = (invw >> (3 * c)) & 7u;
             const bool hit = (keym >> key) & 1u;
             const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
             const uint32_t slot = (hit && key != top) ? sp + pos : (uint32_t)DEPTH;
@@ -293,7 +289,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         } else {
 #pragma unroll
           for (int c = 0; c < 8; c++) {
-            const uint32_t key = CLOSEST ? (invw >> (3 * c)) & 7u : (uint32_t)c;
+            const uint32_t key = (invw >> (3 * c)) & 7u;
             if ((keym >> key) & 1u) {
               if (key == top) {
                 next = ch[c];

@@ -172,7 +172,7 @@ def test_autotune_picks_valid_shape(terrain):
     finally:
         L.rend_hip_free(d_acc)
     batch, refill, best, dflt, variant = list(out)
-    assert refill in (8.0, 16.0, 32.0) and 0 <= variant < 4 and 0 < best <= dflt * 1.0001
+    assert refill in (16.0, 32.0, 48.0) and 0 <= variant < 4 and 0 < best <= dflt * 1.0001
     after, _ = terrain.render_views(v)
     for a, b in zip(before, after):
         np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
@@ -180,5 +180,5 @@ def test_autotune_picks_valid_shape(terrain):
     check(L.rend_hip_get_tuning(tun))
     assert tun[1] == int(refill) and tun[2] == int(variant)
     check(L.rend_hip_set_trace_variant(0))
-    check(L.rend_hip_set_tuning(0, 16))
+    check(L.rend_hip_set_tuning(0, 48))
     check(L.rend_hip_pt_config(4, 3, 0))
