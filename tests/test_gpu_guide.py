@@ -107,23 +107,29 @@ def test_gradient_matches_cpu(guided_scene):
         loss.value) + 1e-3
 
 
-def test_guided_rendering_is_unbiased(guided_scene):
-    """Plain and guided (trained online) images of the occluded-lights
-    terrain converge to the same expectation."""
-    s = guided_scene
+def test_guided_rendering_is_unbiased():
+    """Plain and guided (trained online, one-sample MIS with the BSDF) images
+    converge to the same expectation.  The box scene keeps Monte Carlo noise
+    low enough for a tight bound: 3072 spp per image, 4x4 block means."""
+    s = Scene(config=1, detail=0.25, width=64, height=64)
     L = lib()
-    check(L.rend_hip_guide_enable(s.rd, 0.5, 2, 2, 1e-2, 11), "guide re-init")
-    say("guided render 96 frames x 32 spp")
-    s.L.rend_hip_guide_set(s.rd, 0.5, 2)
-    img_g, _ = render_spp(s, 96)
-    L.rend_hip_guide_disable(s.rd)
-    say("plain render 96 frames x 32 spp")
-    img_p, _ = render_spp(s, 96)
-    blk = lambda x: x.reshape(9, 4, 16, 4, 3).mean(axis=(1, 3))  # noqa: E731
+    try:
+        check(L.rend_hip_guide_enable(s.rd, 0.5, 2, 4, 1e-2, 11), "guide enable")
+        say("guided render 96 frames x 32 spp (box)")
+        img_g, _ = render_spp(s, 96)
+        info = (C.c_float * 4)()
+        check(L.rend_hip_guide_info(s.rd, info))
+        assert info[2] > 0, "the network was trained"
+        L.rend_hip_guide_disable(s.rd)
+        say("plain render 96 frames x 32 spp (box)")
+        img_p, _ = render_spp(s, 96)
+    finally:
+        s.close()
+    blk = lambda x: x.reshape(16, 4, 16, 4, 3).mean(axis=(1, 3))  # noqa: E731
     err = rel_mse(blk(img_g), blk(img_p))
     say(f"block relMSE guided vs plain {err:.2e}, mean {img_g.mean():.4f} vs {img_p.mean():.4f}")
-    assert err < 2e-2
-    assert abs(img_g.mean() / img_p.mean() - 1) < 0.03
+    assert err < 1e-2
+    assert abs(img_g.mean() / img_p.mean() - 1) < 0.02
 
 
 def render_spp(s, frames):
