@@ -155,6 +155,15 @@ int          rend_hip_guide_forward_host(struct rdata *rd, const float *pn,
                                          unsigned int n, float *out);
 int          rend_hip_guide_grad_host(struct rdata *rd, const void *records,
                                       unsigned int n, float *grad, float *loss);
+/* `steps` Adam steps on n host training records (GdRecord layout, target
+ * set), 32K records sampled per step; loss = last step's loss. */
+int          rend_hip_guide_train_host(struct rdata *rd, const void *records,
+                                       unsigned int n, unsigned int steps,
+                                       float *loss);
+/* Mixture records of the rendering-time inference kernel for n
+ * positions+normals: per lobe weight, mean direction (3), concentration. */
+int          rend_hip_guide_mix_host(struct rdata *rd, const float *pn,
+                                     unsigned int n, float *out);
 /* D = A (16x32) * B (32x16), row-major fp32 in, bf16 MFMA inside. */
 int          rend_hip_mfma_check(const float *A, const float *B, float *D);
 

@@ -119,6 +119,9 @@ _SIGS = {
                                               C.c_void_p]),
     "rend_hip_guide_grad_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p,
                                            C.POINTER(C.c_float)]),
+    "rend_hip_guide_mix_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p]),
+    "rend_hip_guide_train_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
+                                            C.POINTER(C.c_float)]),
     "rend_hip_mfma_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rs_new": (C.c_void_p, []),
     "rs_free": (None, [C.c_void_p]),

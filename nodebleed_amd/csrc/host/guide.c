@@ -22,6 +22,11 @@ int rk_guide_eval(const struct PtArgs *a, const void *grid16, const void *w16,
                   const float *master, const void *rays, const void *hits,
                   const unsigned *counts, const void *insts, const void *tris,
                   float *mix, unsigned grid, void *stream);
+int rk_guide_mix(const struct PtArgs *a, const void *grid16, const void *w16,
+                 const float *master, const void *rays, const void *hits,
+                 const unsigned *counts, const void *insts, const void *tris,
+                 const float *pn, unsigned npn, float *mix, unsigned grid,
+                 void *stream);
 int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
                      float *wsum, unsigned cap, void *stream);
 int rk_guide_train(const struct PtArgs *a, const void *grid16, const void *w16,
@@ -38,7 +43,6 @@ int rk_guide_forward(const struct PtArgs *a, const void *grid16,
 int rk_mfma_check(const float *A, const float *B, float *D, void *stream);
 int rk_fill_f32(float *p, unsigned long long n, float v, void *stream);
 
-#define GD_NW16 (GD_HID * GD_IN + 2 * GD_HID * GD_HID + GD_OUT * GD_HID)
 
 static void dfree(void **p)
 {
@@ -102,7 +106,8 @@ static int upload_params(struct rl_guide *g, const float *p, hipStream_t st)
 	 * the mirrors */
 	if (hipMemsetAsync(g->m1, 0, GD_NPARAM * sizeof(float), st) ||
 	    hipMemsetAsync(g->m2, 0, GD_NPARAM * sizeof(float), st) ||
-	    hipMemsetAsync(g->grad, 0, GD_NPARAM * sizeof(float), st))
+	    hipMemsetAsync(g->grad, 0, GD_NPARAM * sizeof(float), st) ||
+	    hipMemsetAsync(g->w16, 0, 2 * GD_NW16 * 2, st))
 		return rl_fail(REND_ENODEV, "guide state clear");
 	int r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16, 1,
 	  0.0f, 0.0f, st);
@@ -125,7 +130,7 @@ static struct rl_guide *guide_alloc(struct rl_scene *s)
 	    (e = hipMalloc((void **)&g->m1, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc((void **)&g->m2, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc(&g->grid16, GD_N_GRID * 2)) ||
-	    (e = hipMalloc(&g->w16, GD_NW16 * 2)) ||
+	    (e = hipMalloc(&g->w16, 2 * GD_NW16 * 2)) ||
 	    (e = hipMalloc((void **)&g->scal, 8 * sizeof(float)))) {
 		s->guide = g;
 		rl_guide_release(s);
@@ -267,8 +272,8 @@ int rl_guide_eval(struct rl_scene *s, const struct PtArgs *a, const void *rays,
                   const void *hits, const unsigned *counts, hipStream_t st)
 {
 	struct rl_guide *g = s->guide;
-	int r = rk_guide_eval(a, g->grid16, g->w16, g->master, rays, hits, counts,
-	  s->insts, s->tris, g->mix, (unsigned)G.cus * 8u, st);
+	int r = rk_guide_mix(a, g->grid16, g->w16, g->master, rays, hits, counts,
+	  s->insts, s->tris, NULL, 0, g->mix, (unsigned)G.cus * 4u, st);
 	return r ? rl_hipfail(r, "guide eval") : 0;
 }
 
@@ -288,7 +293,8 @@ int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths
 		return rl_hipfail(r, "guide targets");
 	const unsigned chunk = 32768;
 	for (unsigned k = 0; k < g->steps; k++) {
-		unsigned first = (k * chunk) % (rc ? rc : 1);
+		/* seed of the step's record sample */
+		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(a, g->grid16, g->w16, g->master, g->rec, counts, rc,
 		  first, chunk, g->scal, g->grad, g->scal + 1, (unsigned)G.cus * 4u, st)))
 			return rl_hipfail(r, "guide train");
@@ -361,6 +367,39 @@ out:
 	return r;
 }
 
+/* Mixture records (GD_MIX floats each) of the inference kernel for n
+ * position/normal pairs. */
+int rend_hip_guide_mix_host(struct rdata *rd, const float *pn, unsigned int n, float *out)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	struct rl_guide *g = s->guide;
+	struct PtArgs a;
+	test_args(g, &a);
+	float *dpn = NULL, *dout = NULL;
+	hipError_t e;
+	int r = 0;
+	if ((e = hipMalloc((void **)&dpn, (size_t)n * 6 * sizeof(float) + 4)) ||
+	    (e = hipMalloc((void **)&dout, (size_t)n * GD_MIX * sizeof(float) + 4)) ||
+	    (e = hipMemcpy(dpn, pn, (size_t)n * 6 * sizeof(float), hipMemcpyHostToDevice))) {
+		r = rl_hipfail(e, "guide mixture buffers");
+		goto out;
+	}
+	if ((r = rk_guide_mix(&a, g->grid16, g->w16, g->master, NULL, NULL, NULL, NULL, NULL,
+	  dpn, n, dout, (unsigned)G.cus * 4u, G.stream))) {
+		r = rl_hipfail(r, "guide mixture");
+		goto out;
+	}
+	if ((e = hipStreamSynchronize(G.stream)) ||
+	    (e = hipMemcpy(out, dout, (size_t)n * GD_MIX * sizeof(float), hipMemcpyDeviceToHost)))
+		r = rl_hipfail(e, "guide mixture");
+out:
+	dfree((void **)&dpn);
+	dfree((void **)&dout);
+	return r;
+}
+
 /* Gradient of the mean weighted loss over n records (GdRecord layout with
  * `target` set) into grad (GD_NPARAM floats); loss receives the loss. */
 int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int n,
@@ -399,6 +438,65 @@ int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int
 	    (e = hipMemcpy(sc, dsc, sizeof(sc), hipMemcpyDeviceToHost)) ||
 	    (e = hipMemset(g->grad, 0, GD_NPARAM * sizeof(float))))
 		r = rl_hipfail(e, "guide grad");
+	if (loss)
+		*loss = sc[1];
+out:
+	dfree(&drec);
+	dfree((void **)&dsc);
+	return r;
+}
+
+/* `steps` training steps (32K records sampled per step, then Adam) on n
+ * host records with `target` set; loss receives the last step's loss. */
+int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned int n,
+                              unsigned int steps, float *loss)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	if (!n)
+		return rl_fail(REND_EINVAL, "no records");
+	struct rl_guide *g = s->guide;
+	struct PtArgs a;
+	test_args(g, &a);
+	void *drec = NULL;
+	float *dsc = NULL;
+	hipError_t e;
+	int r = 0;
+	float sc[2] = {0.0f, 0.0f};
+	const struct GdRecord *rr = records;
+	for (unsigned i = 0; i < n; i++)
+		sc[0] += rr[i].target;
+	if ((e = hipMalloc(&drec, (size_t)n * sizeof(struct GdRecord) + 4)) ||
+	    (e = hipMalloc((void **)&dsc, 2 * sizeof(float))) ||
+	    (e = hipMemcpy(drec, records, (size_t)n * sizeof(struct GdRecord),
+	      hipMemcpyHostToDevice))) {
+		r = rl_hipfail(e, "guide train buffers");
+		goto out;
+	}
+	const unsigned chunk = n < 32768u ? n : 32768u;
+	for (unsigned k = 0; k < steps; k++) {
+		sc[1] = 0.0f;
+		if ((e = hipMemcpyAsync(dsc, sc, sizeof(sc), hipMemcpyHostToDevice, G.stream))) {
+			r = rl_hipfail(e, "guide train");
+			goto out;
+		}
+		unsigned first = (g->step + 1u) * 0x9e3779b9u;
+		if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, NULL, n, first,
+		  chunk, dsc, g->grad, dsc + 1, (unsigned)G.cus * 4u, G.stream))) {
+			r = rl_hipfail(r, "guide train");
+			goto out;
+		}
+		g->step++;
+		if ((r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16,
+		  g->step, g->lr_grid, g->lr_mlp, G.stream))) {
+			r = rl_hipfail(r, "guide adam");
+			goto out;
+		}
+	}
+	if ((e = hipStreamSynchronize(G.stream)) ||
+	    (e = hipMemcpy(sc, dsc, sizeof(sc), hipMemcpyDeviceToHost)))
+		r = rl_hipfail(e, "guide train");
 	if (loss)
 		*loss = sc[1];
 out:

@@ -259,8 +259,8 @@ __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__re
   if ((threadIdx.x & 63u) == 0 && acc > 0.0f) atomicAdd(wsum, acc);
 }
 
-// Forward + backward over records [first, first + n): gradients of the mean
-// of -(target / mean target) log p(dir) into grad (fp32, atomics).
+// Forward + backward over `chunk` records drawn from the batch: gradients of
+// the mean of -(target / mean target) log p(dir) into grad (fp32, atomics).
 __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
                                                     const GdRecord *__restrict__ rec,
                                                     const uint32_t *__restrict__ counts,
@@ -275,7 +275,10 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
   const uint32_t lane = lane_id();
   // records of this step: [first, first + n) within the frame's valid records
   const uint32_t total = counts ? min(counts[PT_CNT_TRAIN], cap) : cap;
-  const uint32_t n = first < total ? min(chunk, total - first) : 0u;
+  // `first` seeds the step's sample: records (i * P + first) mod total for
+  // i < n, with P a prime, are distinct and spread over the whole batch
+  // (records are stored in queue order, i.e. roughly pixel order)
+  const uint32_t n = min(chunk, total);
   const float wmean = wsum[0] / (float)(total ? total : 1u);
   const float inv = wmean > 0.0f ? 1.0f / (wmean * (float)(n ? n : 1u)) : 0.0f;
   float lacc = 0.0f;
@@ -283,7 +286,7 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
     const uint32_t i = base + lane;
     const bool valid = i < n;
     GdRecord r;
-    if (valid) r = rec[first + i];
+    if (valid) r = rec[((uint64_t)i * 2654435761ull + first) % total];
     else {
       for (int k = 0; k < 3; k++) { r.pos[k] = 0.0f; r.nrm[k] = 0.0f; r.dir[k] = 0.0f; }
       r.nrm[2] = 1.0f; r.dir[2] = 1.0f; r.target = 0.0f;
@@ -394,6 +397,205 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
   if (lane == 0 && loss) atomicAdd(loss, lacc);
 }
 
+// ---- register-resident inference ------------------------------------------
+//
+// Each layer is evaluated transposed, Z^T (units x samples) = W (units x K)
+// X^T (K x samples), 16 samples per wave: the MFMA B operand of lane l holds
+// 8 consecutive inputs (k = 8 (l/16) .. +7) of sample l % 16 and its D
+// result holds units 4 (l/16) + r (r = 0..3) of each 16-unit tile for the
+// same sample.  Choosing the next layer's column order so that input k of
+// block kb is unit 32 kb + 16 (j/4) + 4 (l/16) + j%4 (j = k % 8) turns the D
+// registers of tiles 2kb and 2kb+1 directly into that block's B operand:
+// activations never leave the registers.  The output layer's rows are
+// ordered so that lane group g holds lobes 2g and 2g+1 (10 outputs).
+// The permuted bf16 copy of the weights lives at w16 + GD_NW16.
+
+// Position of hidden unit u in the permuted column order.
+__host__ __device__ __forceinline__ uint32_t perm_col(uint32_t u) {
+  return 32u * (u >> 5) + 8u * ((u & 15u) >> 2) + 4u * ((u & 31u) >> 4) + (u & 3u);
+}
+
+// Permuted row of output o (o < 40 used): lobe pair g = o / 10, slot
+// s = o % 10 in tile s / 4, register s % 4.
+__host__ __device__ __forceinline__ uint32_t perm_out_row(uint32_t o) {
+  const uint32_t g = o / 10u, sl = o % 10u;
+  return 16u * (sl >> 2) + 4u * g + (sl & 3u);
+}
+
+// Index of master parameter i in the permuted mirror, or -1.
+__device__ __forceinline__ int perm_dst(uint32_t i) {
+  if (i >= GD_OFS_W1 && i < GD_OFS_B1) return (int)(i - GD_OFS_W1);
+  if (i >= GD_OFS_W2 && i < GD_OFS_B2) {
+    const uint32_t k = i - GD_OFS_W2;
+    return (int)(GD_HID * GD_IN + (k / GD_HID) * GD_HID + perm_col(k % GD_HID));
+  }
+  if (i >= GD_OFS_W3 && i < GD_OFS_B3) {
+    const uint32_t k = i - GD_OFS_W3;
+    return (int)(GD_HID * GD_IN + GD_HID * GD_HID + (k / GD_HID) * GD_HID + perm_col(k % GD_HID));
+  }
+  if (i >= GD_OFS_W4 && i < GD_OFS_B4) {
+    const uint32_t k = i - GD_OFS_W4, o = k / GD_HID;
+    if (o >= GD_MIX) return -1;
+    return (int)(GD_HID * GD_IN + 2 * GD_HID * GD_HID + perm_out_row(o) * GD_HID +
+                 perm_col(k % GD_HID));
+  }
+  return -1;
+}
+
+// bias + ReLU of tiles 2kb, 2kb+1 packed as the next layer's B operand.
+__device__ __forceinline__ bf16x8 relu_pack(const f32x4 &d0, const f32x4 &d1, const float *b,
+                                            uint32_t t0, uint32_t g) {
+  bf16x8 x;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    x[r] = (__bf16)fmaxf(d0[r] + b[16u * t0 + 4u * g + (uint32_t)r], 0.0f);
+    x[4 + r] = (__bf16)fmaxf(d1[r] + b[16u * (t0 + 1u) + 4u * g + (uint32_t)r], 0.0f);
+  }
+  return x;
+}
+
+// One hidden layer with K = 64: 4 output tiles from two B blocks.
+__device__ __forceinline__ void layer64(const __bf16 *__restrict__ W, const bf16x8 xb[2],
+                                        uint32_t s, uint32_t g, f32x4 h[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+      const bf16x8 av = *(const bf16x8 *)(W + (16u * t + s) * GD_HID + 32u * kb + 8u * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xb[kb], acc, 0, 0, 0);
+    }
+    h[t] = acc;
+  }
+}
+
+// Mixture records (GD_MIX floats) of queued hit points (pn == nullptr) or of
+// n position/normal pairs pn[6 i ..].
+__global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__restrict__ grid,
+                                                   const __bf16 *__restrict__ wp,
+                                                   const float *__restrict__ master,
+                                                   const rray *__restrict__ rays,
+                                                   const rhit *__restrict__ hits,
+                                                   const uint32_t *__restrict__ counts,
+                                                   const dinst *__restrict__ insts,
+                                                   const RTri *__restrict__ tris,
+                                                   const float *__restrict__ pn, uint32_t npn,
+                                                   float *__restrict__ mix) {
+  const uint32_t n = pn ? npn : counts[PT_CNT_CUR];
+  const uint32_t lane = lane_id();
+  const uint32_t s = lane & 15u, g = lane >> 4;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  for (uint32_t base = wave * 16u; base < n; base += nwaves * 16u) {
+    // The weight fragments are loop invariant; an opaque zero offset keeps
+    // the compiler from hoisting all 26 of them into registers (they are
+    // re-read from the caches per 16 samples instead of spilling).
+    uint32_t wofs = 0;
+    asm volatile("" : "+s"(wofs));
+    const __bf16 *W1 = wp + wofs, *W2 = W1 + GD_HID * GD_IN, *W3 = W2 + GD_HID * GD_HID,
+                 *W4 = W3 + GD_HID * GD_HID;
+    const uint32_t i = base + s;
+    float p[3] = {0, 0, 0}, nn[3] = {0, 0, 1};
+    bool ok = false;
+    if (i < n) {
+      if (pn) {
+        for (int k = 0; k < 3; k++) { p[k] = pn[6 * i + k]; nn[k] = pn[6 * i + 3 + k]; }
+        ok = true;
+      } else {
+        ok = hit_point(rays[i], hits[i], insts, tris, p, nn);
+      }
+    }
+    // layer-1 B operand: inputs 8g .. 8g+7 of sample s
+    bf16x8 x0;
+    if (g < 3u) {
+      float q[3];
+      normalise_pos(a, p, q);
+#pragma unroll
+      for (int lv = 0; lv < 4; lv++) {
+        const int l = 4 * (int)g + lv;
+        float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          uint32_t idx;
+          float w;
+          grid_corner(l, q, c, idx, w);
+          const __half2 gv = *(const __half2 *)&grid[((size_t)l * GD_TABLE + idx) * GD_FEAT];
+          f0 += w * __low2float(gv);
+          f1 += w * __high2float(gv);
+        }
+        x0[2 * lv] = (__bf16)f0;
+        x0[2 * lv + 1] = (__bf16)f1;
+      }
+    } else {
+      x0[0] = (__bf16)nn[0];
+      x0[1] = (__bf16)nn[1];
+      x0[2] = (__bf16)nn[2];
+      x0[3] = (__bf16)1.0f;
+      x0[4] = x0[5] = x0[6] = x0[7] = (__bf16)0.0f;
+    }
+    // layer 1 (K = 32)
+    f32x4 h[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const bf16x8 av = *(const bf16x8 *)(W1 + (16u * t + s) * GD_IN + 8u * g);
+      const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+      h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, x0, z, 0, 0, 0);
+    }
+    bf16x8 xb[2];
+    xb[0] = relu_pack(h[0], h[1], master + GD_OFS_B1, 0u, g);
+    xb[1] = relu_pack(h[2], h[3], master + GD_OFS_B1, 2u, g);
+    layer64(W2, xb, s, g, h);
+    xb[0] = relu_pack(h[0], h[1], master + GD_OFS_B2, 0u, g);
+    xb[1] = relu_pack(h[2], h[3], master + GD_OFS_B2, 2u, g);
+    layer64(W3, xb, s, g, h);
+    xb[0] = relu_pack(h[0], h[1], master + GD_OFS_B3, 0u, g);
+    xb[1] = relu_pack(h[2], h[3], master + GD_OFS_B3, 2u, g);
+    // output layer: tiles 0..2 hold this lane group's 10 outputs
+    float o[12];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++) {
+        const bf16x8 av = *(const bf16x8 *)(W4 + (16u * t + s) * GD_HID + 32u * kb + 8u * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, xb[kb], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) o[4 * t + r] = acc[r];
+    }
+#pragma unroll
+    for (int sl = 0; sl < 10; sl++) o[sl] += master[GD_OFS_B4 + 10u * g + (uint32_t)sl];
+    // softmax over the 8 lobes: the sample's 4 lanes are s, s+16, s+32, s+48
+    float mx = fmaxf(o[0], o[5]);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
+    float sum = e0 + e1;
+    sum += __shfl_xor(sum, 16);
+    sum += __shfl_xor(sum, 32);
+    float rec[10];
+#pragma unroll
+    for (int lb = 0; lb < 2; lb++) {
+      const float *ol = &o[5 * lb];
+      float x = ol[1], y = ol[2], z = ol[3];
+      const float l2 = x * x + y * y + z * z;
+      if (l2 < 1e-12f) { x = 0.0f; y = 0.0f; z = 1.0f; }
+      else { const float il = 1.0f / sqrtf(l2); x *= il; y *= il; z *= il; }
+      const float lk = fminf(fmaxf(ol[4], GD_LOGK_MIN), GD_LOGK_MAX);
+      rec[5 * lb] = (lb ? e1 : e0) / sum;
+      rec[5 * lb + 1] = x;
+      rec[5 * lb + 2] = y;
+      rec[5 * lb + 3] = z;
+      rec[5 * lb + 4] = expf(lk);
+    }
+    if (i < n && ok) {
+      float2 *dst = (float2 *)&mix[(size_t)i * GD_MIX + 10u * g];
+#pragma unroll
+      for (int k = 0; k < 5; k++) dst[k] = make_float2(rec[2 * k], rec[2 * k + 1]);
+    }
+  }
+}
+
 // Adam on the fp32 master parameters; refreshes the fp16/bf16 mirrors and
 // clears the gradient.
 __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ grad,
@@ -428,6 +630,8 @@ __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ gra
     else if (i >= GD_OFS_W3 && i < GD_OFS_B3) dst = GD_HID * GD_IN + GD_HID * GD_HID + (int)(i - GD_OFS_W3);
     else if (i >= GD_OFS_W4 && i < GD_OFS_B4) dst = GD_HID * GD_IN + 2 * GD_HID * GD_HID + (int)(i - GD_OFS_W4);
     if (dst >= 0) w16[dst] = (__bf16)master[i];
+    const int pd = perm_dst(i);
+    if (pd >= 0) w16[GD_NW16 + pd] = (__bf16)master[i];
   }
 }
 
@@ -498,6 +702,18 @@ extern "C" int rk_guide_eval(const PtArgs *a, const void *grid16, const void *w1
   hipLaunchKernelGGL(k_guide_eval, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
                      mknet(grid16, w16, master), (const rray *)rays, (const rhit *)hits,
                      counts, (const dinst *)insts, (const RTri *)tris, mix);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_mix(const PtArgs *a, const void *grid16, const void *w16,
+                            const float *master, const void *rays, const void *hits,
+                            const unsigned *counts, const void *insts, const void *tris,
+                            const float *pn, unsigned npn, float *mix, unsigned grid,
+                            void *stream) {
+  hipLaunchKernelGGL(k_guide_mix, dim3(grid), dim3(256), 0, (hipStream_t)stream, *a,
+                     (const __half *)grid16, (const __bf16 *)w16 + GD_NW16, master,
+                     (const rray *)rays, (const rhit *)hits, counts, (const dinst *)insts,
+                     (const RTri *)tris, pn, npn, mix);
   return (int)hipGetLastError();
 }
 

@@ -79,6 +79,9 @@ struct PtLight {
 #define GD_OFS_B4  (GD_OFS_W4 + GD_OUT * GD_HID)
 #define GD_NPARAM  (GD_OFS_B4 + GD_OUT)
 #define GD_NMLP    (GD_NPARAM - GD_N_GRID)
+/* bf16 weight mirror: W1..W4 packed in natural order, then the same in the
+ * permuted order of the register-resident inference kernel */
+#define GD_NW16    (GD_HID * GD_IN + 2 * GD_HID * GD_HID + GD_OUT * GD_HID)
 
 /* Per shading point mixture: for each lobe weight, mean xyz, kappa. */
 #define GD_MIX     (5 * GD_LOBES)
