@@ -81,6 +81,11 @@ int          rend_hip_set_trace_variant(int variant);
 /* out: {blocks per CU, refill, variant} currently in use. */
 int          rend_hip_get_tuning(unsigned int out[3]);
 
+/* Traversal step limit per ray (default 4,194,304); a ray that reaches it,
+ * or overflows the 64-entry stack, ends with REND_EOVERFLOW reported by the
+ * next synchronous call together with that ray's record. */
+int          rend_hip_set_iter_limit(unsigned int steps);
+
 /* Builder hazards (bit set) met while building rd's trees; nonzero means the
  * reference's own build would be undefined or degenerate for this scene. */
 unsigned int rend_hip_hazards(struct rdata *rd);

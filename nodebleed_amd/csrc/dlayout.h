@@ -78,6 +78,10 @@ struct CamArgs {
 #define RK_CTL_WORDS   (RK_CTL_ERR + RK_SEG_STRIDE)
 
 /* Error bits written by kernels into the device status word. */
+/* Diagnostic block shared by every traversal launch (after the control
+ * ring): [0] sticky error bits, [1] claim flag, [4..11] the first failing
+ * ray's record, [12] its index, [13] its iteration count, [14] its error. */
+#define RK_DIAG_WORDS 16
 #define RK_ERR_STACK  1u
 #define RK_ERR_ITER   2u
 

@@ -22,7 +22,8 @@ struct rl_global {
 	unsigned            max_iter;    /* traversal step limit per ray */
 	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
 	int                 variant;     /* traversal occupancy variant */
-	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS */
+	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS, then RK_DIAG_WORDS */
+	unsigned           *diag;        /* error block shared by all traversal launches */
 	unsigned            ctl_next;
 	unsigned            last_slot;
 	unsigned long long *stats;       /* 4 counters of the counting variant */
@@ -115,6 +116,6 @@ int rk_trace_launch(const void *arena, unsigned long long tlas,
                     void *spill, unsigned long long *stats, int closest,
                     int count, int rcp_ieee, int variant, unsigned int grid,
                     unsigned int refill, unsigned int max_iter,
-                    const unsigned int *n_dev, void *stream);
+                    const unsigned int *n_dev, unsigned int *diag, void *stream);
 
 #endif

@@ -86,15 +86,15 @@ int rl_device(void)
 	G.max_iter = 1u << 22;
 	G.grid_mult = 0;
 	G.variant = 0;
-	if ((e = hipMalloc((void **)&G.ctl, RL_CTL_SLOTS * RK_CTL_WORDS * sizeof(unsigned)))
-	      != hipSuccess ||
-	    (e = hipMemset(G.ctl, 0, RL_CTL_SLOTS * RK_CTL_WORDS * sizeof(unsigned)))
-	      != hipSuccess ||
+	const size_t ctl_words = RL_CTL_SLOTS * RK_CTL_WORDS + RK_DIAG_WORDS;
+	if ((e = hipMalloc((void **)&G.ctl, ctl_words * sizeof(unsigned))) != hipSuccess ||
+	    (e = hipMemset(G.ctl, 0, ctl_words * sizeof(unsigned))) != hipSuccess ||
 	    (e = hipMalloc((void **)&G.stats, 4 * sizeof(unsigned long long)))
 	      != hipSuccess) {
 		pthread_mutex_unlock(&g_lock);
 		return rl_hipfail(e, "control words");
 	}
+	G.diag = G.ctl + RL_CTL_SLOTS * RK_CTL_WORDS;
 	G.ready = 1;
 	pthread_mutex_unlock(&g_lock);
 	return 0;
@@ -695,7 +695,7 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
 	  d_occ, n, G.ctl + RK_CTL_WORDS * slot, G.spill, count ? G.stats : NULL,
-	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, st);
+	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, G.diag, st);
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;
@@ -723,16 +723,29 @@ int rl_check_errors(hipStream_t st)
 	hipError_t e = hipStreamSynchronize(st);
 	if (e != hipSuccess)
 		return rl_hipfail(e, "trace");
-	unsigned w[2];
-	e = hipMemcpy(w, G.ctl + RK_CTL_WORDS * G.last_slot + RK_CTL_ERR - 1, sizeof(w), hipMemcpyDeviceToHost);
+	unsigned w[RK_DIAG_WORDS];
+	e = hipMemcpy(w, G.diag, sizeof(w), hipMemcpyDeviceToHost);
 	if (e != hipSuccess)
 		return rl_hipfail(e, "trace status");
-	if (w[1]) {
-		hipMemset(G.ctl + RK_CTL_WORDS * G.last_slot + RK_CTL_ERR, 0, sizeof(unsigned));
-		return rl_fail(REND_EOVERFLOW, "traversal %s",
-		  (w[1] & RK_ERR_STACK) ? "stack overflow (64 entries)" :
-		  "iteration limit");
-	}
+	if (!w[0])
+		return 0;
+	/* report and clear: the first failing ray of all launches since the
+	 * last check */
+	hipMemset(G.diag, 0, sizeof(w));
+	float r[8];
+	memcpy(r, &w[4], sizeof(r));
+	return rl_fail(REND_EOVERFLOW, "traversal %s (first: ray %u o=(%.9g %.9g %.9g) "
+	  "d=(%.9g %.9g %.9g) tmax=%.9g, %u steps)",
+	  (w[14] & RK_ERR_STACK) ? "stack overflow (64 entries)" : "iteration limit",
+	  w[12], (double)r[0], (double)r[1], (double)r[2], (double)r[4], (double)r[5],
+	  (double)r[6], (double)r[3], w[13]);
+}
+
+int rend_hip_set_iter_limit(unsigned int steps)
+{
+	if (steps < 64)
+		return rl_fail(REND_EINVAL, "iteration limit %u", steps);
+	G.max_iter = steps;
 	return 0;
 }
 

@@ -80,6 +80,19 @@ __device__ __forceinline__ float rcp_table(float x, const uint16_t *tab) {
   return __uint_as_float(r);
 }
 
+// Sticky error bits for the host, and the first failing ray's record.
+__device__ __noinline__ void report_error(uint32_t *diag, uint32_t code, const rray *rays,
+                                          uint32_t rid, uint32_t iter) {
+  atomicOr(&diag[0], code);
+  if (atomicCAS(&diag[1], 0u, 1u) == 0u) {
+    const uint4 *r = (const uint4 *)&rays[rid];
+    uint4 *d = (uint4 *)&diag[4];
+    d[0] = r[0];
+    d[1] = r[1];
+    d[2] = make_uint4(rid, iter, code, 0u);
+  }
+}
+
 // Loads a node and runs the slab test of its 8 children: entry distances
 // into tmn, child words into ch, the octant's rank word into invw; returns
 // the hit mask indexed by rank.  FAST: every slab distance is known to be
@@ -146,7 +159,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              rhit *__restrict__ hits, uint8_t *__restrict__ occ, uint32_t n,
              uint32_t *__restrict__ ctl, uint2 *__restrict__ spill,
              unsigned long long *__restrict__ stats, uint32_t refill,
-             uint32_t max_iter, const uint32_t *__restrict__ n_dev) {
+             uint32_t max_iter, const uint32_t *__restrict__ n_dev,
+             uint32_t *__restrict__ diag) {
   __shared__ uint16_t s_tab[2048];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
   // stack rows [0, DEPTH) per lane, plus one junk row that branch-free
@@ -244,7 +258,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     bool need_pop = false;
     bool done = false;
     if (++iter > max_iter) {
-      atomicOr(&ctl[RK_CTL_ERR], RK_ERR_ITER);
+      report_error(diag, RK_ERR_ITER, rays, rid, iter);
       done = true;
     } else if (!(ofs & DNODE_LEAF)) {
       const dnode *nd = (const dnode *)(arena + base + ofs);
@@ -262,7 +276,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       } else if (sp + hitcnt > REND_STACK_MAX) {
         // The reference asserts here (rend.c:834); report instead of writing
         // past the stack.
-        atomicOr(&ctl[RK_CTL_ERR], RK_ERR_STACK);
+        report_error(diag, RK_ERR_STACK, rays, rid, iter);
         done = true;
       } else {
         // Highest rank among hits (nearest) is the next node, the others are
@@ -457,13 +471,14 @@ struct LaunchArgs {
   unsigned long long *stats;
   uint32_t refill, max_iter;
   const uint32_t *n_dev;
+  uint32_t *diag;
 };
 
 template <bool C, bool N, bool I, int D, int W>
 hipError_t launch_one(uint32_t grid, hipStream_t s, const LaunchArgs &a) {
   hipLaunchKernelGGL((k_trace<C, N, I, D, W>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
                      a.arena, a.tlas, a.insts, a.rays, a.hits, a.occ, a.n, a.ctl,
-                     a.spill, a.stats, a.refill, a.max_iter, a.n_dev);
+                     a.spill, a.stats, a.refill, a.max_iter, a.n_dev, a.diag);
   return hipGetLastError();
 }
 
@@ -537,7 +552,7 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
                                int closest, int count, int rcp_ieee, int variant,
                                unsigned int grid, unsigned int refill,
                                unsigned int max_iter, const unsigned int *n_dev,
-                               void *stream) {
+                               unsigned int *diag, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_u32, dim3(4), dim3(64), 0, s, ctl, (uint32_t)RK_CTL_ERR);
   hipError_t e = hipGetLastError();
@@ -557,6 +572,7 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
   a.refill = refill;
   a.max_iter = max_iter;
   a.n_dev = n_dev;
+  a.diag = diag;
   const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
   if (count) {
     switch (k) {
