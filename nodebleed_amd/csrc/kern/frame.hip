@@ -36,6 +36,15 @@ struct RMtl {  // rend.h:54-62
 
 namespace {
 
+// A direction component of exactly +-0 makes 1/d and o/d infinite; under the
+// reference's slab rules (x86 MAXPS/MINPS, NaN operand wins) the NaN slab
+// distances let every box test pass and the ray visits the whole tree.
+// Generated directions are nudged to +-1e-20 instead (measure zero).
+__device__ __forceinline__ float nonzero_dir(float x) {
+  return x == 0.0f ? __builtin_copysignf(1e-20f, x) : x;
+}
+
+
 __device__ __forceinline__ void camray(const CamArgs &c, float px, float py,
                                        float u0, float u1, float u2, float u3,
                                        rray &r) {
@@ -87,7 +96,7 @@ __global__ void k_camrays(CamArgs c, uint32_t rw, uint32_t rh, uint32_t spp,
          rng_u01(key, 2), rng_u01(key, 3), r);
   float4 *o = (float4 *)&out[i];
   o[0] = make_float4(r.ox, r.oy, r.oz, r.tmax);
-  o[1] = make_float4(r.dx, r.dy, r.dz, r.pad);
+  o[1] = make_float4(nonzero_dir(r.dx), nonzero_dir(r.dy), nonzero_dir(r.dz), r.pad);
 }
 
 __device__ __forceinline__ uint32_t pack8(float a) {
@@ -198,7 +207,7 @@ __global__ void k_diffuse_rays(const rray *__restrict__ prim,
   }
   float4 *po = (float4 *)&out[i];
   po[0] = make_float4(o.ox, o.oy, o.oz, o.tmax);
-  po[1] = make_float4(o.dx, o.dy, o.dz, o.pad);
+  po[1] = make_float4(nonzero_dir(o.dx), nonzero_dir(o.dy), nonzero_dir(o.dz), o.pad);
 }
 
 // Plain streaming copy (16 bytes per lane per step): the reference point

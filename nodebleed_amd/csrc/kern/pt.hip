@@ -75,12 +75,22 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
   return want ? base + below : 0xffffffffu;
 }
 
+// A direction component of exactly +-0 gives infinite 1/d and o/d; under the
+// reference's slab rules (x86 MAXPS/MINPS, NaN operand wins) the resulting
+// NaN distances make every box test pass, and such a ray visits the whole
+// tree.  Sampled directions hit exact zeros with probability ~2^-24 per
+// draw (more often next to axis-aligned normals); nudging the component to
+// +-1e-20 is a measure-zero change of the sampled distribution.
+__device__ __forceinline__ float nonzero_dir(float x) {
+  return x == 0.0f ? __builtin_copysignf(1e-20f, x) : x;
+}
+
 __device__ __forceinline__ void store_ray(rray *out, uint32_t i, float ox, float oy,
                                           float oz, float tmax, float dx, float dy,
                                           float dz) {
   float4 *o = (float4 *)&out[i];
   o[0] = make_float4(ox, oy, oz, tmax);
-  o[1] = make_float4(dx, dy, dz, 0.0f);
+  o[1] = make_float4(nonzero_dir(dx), nonzero_dir(dy), nonzero_dir(dz), 0.0f);
 }
 
 // Batch layout: path index -> (view, pixel, sample) by the view table's
@@ -155,18 +165,59 @@ __global__ void k_pt_gen(PtArgs a, const PtView *__restrict__ views,
   qpath[i] = i;
 }
 
-// Uniform point on a light triangle chosen by the area*luminance CDF.
+constexpr uint32_t kLightScan = 64;  // up to this many emitters: per-point choice
+constexpr float kLightFloor = 0.1f;  // share of the area*luminance choice kept
+
+// Importance of light L for a receiver at p with normal n: power times the
+// two cosines at the triangle's centroid over the squared distance.
+__device__ __forceinline__ float light_importance(const PtLight &L, const float p[3],
+                                                  const float n[3]) {
+  float d[3];
+  for (int j = 0; j < 3; j++) d[j] = L.v0[j] + (L.e0[j] + L.e1[j]) * (1.0f / 3.0f) - p[j];
+  const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  const float cr = fmaxf(n[0] * d[0] + n[1] * d[1] + n[2] * d[2], 0.0f);
+  const float cl = fmaxf(-(L.n[0] * d[0] + L.n[1] * d[1] + L.n[2] * d[2]), 0.0f);
+  const float lum = 0.2126f * L.E[0] + 0.7152f * L.E[1] + 0.0722f * L.E[2];
+  // cr cl / d2 is cos_r cos_l d2 / d2^2; area keeps small close lights finite
+  const float dd = fmaxf(d2, L.area);
+  return lum * L.area * cr * cl / (dd * dd);
+}
+
+// Uniform point on an emissive triangle.  With few emitters the triangle is
+// chosen per receiver: probability (1 - kLightFloor) * importance / total +
+// kLightFloor * (area*luminance share), so every emitter keeps a nonzero
+// probability and the estimator stays unbiased; with many emitters, by the
+// area*luminance CDF alone.
 __device__ __forceinline__ void sample_light(const PtLight *__restrict__ lights,
                                              const float *__restrict__ cdf,
-                                             uint32_t nl, float u0, float u1, float u2,
+                                             uint32_t nl, const float p[3], const float n[3],
+                                             float u0, float u1, float u2,
                                              float q[3], float nl3[3], float E[3],
                                              float &pdf_area) {
-  uint32_t lo = 0, hi = nl;
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (cdf[mid] <= u0) lo = mid + 1; else hi = mid;
+  uint32_t k = 0;
+  float pk = 0.0f;
+  if (nl <= kLightScan) {
+    float tot = 0.0f;
+    for (uint32_t i = 0; i < nl; i++) tot += light_importance(lights[i], p, n);
+    const float wi = tot > 0.0f ? (1.0f - kLightFloor) / tot : 0.0f;
+    const float wb = tot > 0.0f ? kLightFloor : 1.0f;
+    float acc = 0.0f;
+    k = nl - 1;
+    for (uint32_t i = 0; i < nl; i++) {
+      const float pi = wi * light_importance(lights[i], p, n) + wb * lights[i].pick;
+      acc += pi;
+      if (u0 < acc) { k = i; pk = pi; break; }
+      pk = pi;
+    }
+  } else {
+    uint32_t lo = 0, hi = nl;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (cdf[mid] <= u0) lo = mid + 1; else hi = mid;
+    }
+    k = lo < nl ? lo : nl - 1;
+    pk = lights[k].pick;
   }
-  const uint32_t k = lo < nl ? lo : nl - 1;
   const PtLight &L = lights[k];
   const float su = sqrtf(u1);
   const float b0 = su * (1.0f - u2), b1 = su * u2;  // uniform on the triangle
@@ -175,7 +226,7 @@ __device__ __forceinline__ void sample_light(const PtLight *__restrict__ lights,
     nl3[j] = L.n[j];
     E[j] = L.E[j];
   }
-  pdf_area = L.pick / L.area;
+  pdf_area = pk / L.area;
 }
 
 __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
@@ -249,7 +300,8 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
           // next-event estimation
           if (a.nlights) {
             float q[3], ln[3], E[3], pdfa;
-            sample_light(lights, lcdf, a.nlights, rng_u01(key, dbase + 0),
+            const float sp[3] = {px, py, pz}, sn[3] = {nx, ny, nz};
+            sample_light(lights, lcdf, a.nlights, sp, sn, rng_u01(key, dbase + 0),
                          rng_u01(key, dbase + 1), rng_u01(key, dbase + 2), q, ln, E, pdfa);
             float lx = q[0] - px, ly = q[1] - py, lz = q[2] - pz;
             const float d2 = lx * lx + ly * ly + lz * lz;
