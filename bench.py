@@ -247,42 +247,24 @@ def bench_views(a):
 
 
 def bench_guide(a):
-    """Equal-time comparison on the occluded-lights terrain (8 emitters behind
-    walls), 1920x1080: relMSE of plain and guided path tracing against a
-    high-spp plain reference, as a function of wall time."""
+    """Equal-error wall time on the occluded-lights scene: plain path tracing
+    runs `guide_frames` frames (time T_p, relMSE E_p against a high-spp plain
+    GPU reference); guided path tracing (trained online from its own paths,
+    starting from an untrained network) runs until its relMSE first reaches
+    E_p (time T_g, checked every few frames with the check itself excluded
+    from the clock).  Value: T_g / T_p."""
     from nodebleed_amd import REND_MODE_PATHTRACE
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import rel_mse
     L = lib()
     w, h = (W, H) if not a.guide_small else (480, 270)
-    spf = 4
+    spf = a.guide_spf
     scene = Scene(config=5, seed=a.seed, detail=1.0, width=w, height=h, spp=spf)
     L.rend_staticinit(32, 8)
     check(L.rend_hip_set_mode(REND_MODE_PATHTRACE), "mode")
     maxdepth, rr = 6, 3
     check(L.rend_hip_pt_config(maxdepth, rr, 0), "pt config")
-
-    def run(nframes, guided, sample0=0, curve_every=0, ref=None):
-        L.rend_clraccum(scene.rd)
-        if guided:
-            check(L.rend_hip_guide_enable(scene.rd, 0.5, 4, a.guide_steps, a.guide_lr, 7),
-                  "guide")
-        else:
-            L.rend_hip_guide_disable(scene.rd)
-        curve = []
-        t = time.perf_counter()
-        samples = sample0
-        for k in range(nframes):
-            L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
-            nr = C.c_uint(0)
-            check(L.rend_hip_render_frame(scene.rd, C.byref(nr)), "render")
-            samples += spf
-            if curve_every and (k + 1) % curve_every == 0:
-                el = time.perf_counter() - t
-                img = download(samples - sample0)
-                curve.append((el, (k + 1) * spf, rel_mse(img, ref) if ref is not None else None))
-                t = time.perf_counter() - el
-        el = time.perf_counter() - t
-        return el, download(samples - sample0), curve
+    cfg = (a.guide_mix, a.guide_stride, a.guide_steps, a.guide_lr)
 
     def download(nspp):
         check(L.rend_hip_download_acc(scene.rd), "download")
@@ -290,38 +272,63 @@ def bench_guide(a):
         acc = scene._arr(v.acc, np.float32, w * h * 3).reshape(h, w, 3)
         return acc.astype(np.float64) / nspp
 
-    from oracle_lib import rel_mse
+    def run(nframes, guided, sample0=0, ref=None, check_every=0, stop_at=None, tmax=None,
+            tag=""):
+        L.rend_clraccum(scene.rd)
+        if guided:
+            check(L.rend_hip_guide_enable(scene.rd, *cfg, 7), "guide")
+        else:
+            L.rend_hip_guide_disable(scene.rd)
+        curve, el, samples, k = [], 0.0, sample0, 0
+        while k < nframes:
+            t = time.perf_counter()
+            L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
+            nr = C.c_uint(0)
+            check(L.rend_hip_render_frame(scene.rd, C.byref(nr)), "render")
+            el += time.perf_counter() - t
+            samples += spf
+            k += 1
+            if check_every and k % check_every == 0:
+                err = rel_mse(download(k * spf), ref)
+                curve.append((round(el, 3), k * spf, err))
+                log(f"  {tag} {k * spf} spp {el:.2f} s relMSE {err:.4f}")
+                if stop_at is not None and err <= stop_at:
+                    break
+                if tmax is not None and el > tmax:
+                    break
+            elif k % 256 == 0:
+                log(f"  {tag} frame {k}/{nframes} {el:.1f} s")
+        return el, download(k * spf), curve
+
     t = time.perf_counter()
-    _, ref, _ = run(a.guide_ref_frames, False, sample0=1 << 20)
+    _, ref, _ = run(a.guide_ref_frames, False, sample0=1 << 24, tag="reference")
     log(f"reference: {a.guide_ref_frames * spf} spp plain in {time.perf_counter() - t:.1f} s")
-    every = max(1, a.guide_frames // 16)
-    tp, plain, cp = run(a.guide_frames, False, curve_every=every, ref=ref)
+    every = max(1, a.guide_frames // 8)
+    tp, plain, cp = run(a.guide_frames, False, ref=ref, check_every=every, tag="plain")
     err_p = rel_mse(plain, ref)
-    log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.3e}")
-    tg, guided, cg = run(a.guide_frames, True, curve_every=every, ref=ref)
+    log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.4f}")
+    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 2), stop_at=err_p,
+                         tmax=3.0 * tp, tag="guided")
     err_g = rel_mse(guided, ref)
     info = (C.c_float * 4)()
     L.rend_hip_guide_info(scene.rd, info)
-    log(f"guided: {a.guide_frames * spf} spp in {tg:.2f} s, relMSE {err_g:.3e}, "
-        f"loss {info[0]:.3f}")
-    for (t1, n1, e1), (t2, n2, e2) in zip(cp, cg):
-        log(f"  spp {n1}: plain {t1:.2f} s {e1:.3e} | guided {t2:.2f} s {e2:.3e}")
-    # wall time at which guided first reaches plain's final error
-    t_eq = next((tt for tt, _, e in cg if e <= err_p), None)
-    ratio = t_eq / tp if t_eq is not None else None
-    log(f"time for guided to reach plain's final relMSE: {t_eq} s -> ratio {ratio}")
+    reached = err_g <= err_p
+    ratio = tg / tp if reached else float("nan")
+    log(f"guided: relMSE {err_g:.4f} after {tg:.2f} s ({'reached' if reached else 'not reached'}"
+        f" plain's {err_p:.4f}); ratio {ratio:.3f}")
     L.rend_hip_guide_disable(scene.rd)
-    emit("guided/plain wall-time ratio at equal relMSE", ratio if ratio else float("nan"),
-         "ratio", a.guide_frames, 0, tp * 1e3 / a.guide_frames,
-         {"workload": f"occluded-lights terrain (1M tris + 8 walls + 8 emitters), {w}x{h}, "
-                      f"{spf} spp/frame, max depth {maxdepth}, RR after {rr}; reference "
-                      f"{a.guide_ref_frames * spf} spp plain (GPU)",
-          "plain": {"seconds": round(tp, 3), "relmse": err_p,
-                    "curve": [(round(x, 3), n, e) for x, n, e in cp]},
-          "guided": {"seconds": round(tg, 3), "relmse": err_g,
-                     "curve": [(round(x, 3), n, e) for x, n, e in cg]},
-          "guide": {"mix": 0.5, "train_stride": 4, "steps_per_batch": a.guide_steps,
-                    "lr": a.guide_lr}},
+    emit("guided/plain wall time to equal relMSE", ratio, "ratio", a.guide_frames, 0,
+         tp * 1e3 / a.guide_frames,
+         {"workload": f"occluded-lights terrain (1M tris, 8 emitters in chimneys lighting ceiling "
+                      f"spots, terrain lit indirectly), {w}x{h}, {spf} spp/frame, max depth "
+                      f"{maxdepth}, RR after {rr}; reference {a.guide_ref_frames * spf} spp plain "
+                      f"(GPU)",
+          "plain": {"seconds": round(tp, 3), "spp": a.guide_frames * spf, "relmse": err_p,
+                    "curve": cp},
+          "guided": {"seconds": round(tg, 3), "relmse": err_g, "reached": reached,
+                     "curve": cg, "adam_steps": int(info[2])},
+          "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_batch": cfg[2],
+                    "lr": cfg[3]}},
          higher=False)
     scene.close()
     return 0
@@ -346,9 +353,12 @@ def main():
                          "launch; guide: equal-time guided vs plain path tracing")
     ap.add_argument("--guide-frames", type=int, default=64)
     ap.add_argument("--guide-ref-frames", type=int, default=1024)
-    ap.add_argument("--guide-steps", type=int, default=2)
+    ap.add_argument("--guide-spf", type=int, default=16, help="samples per pixel per frame")
+    ap.add_argument("--guide-mix", type=float, default=0.5)
+    ap.add_argument("--guide-stride", type=int, default=2)
+    ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
-    ap.add_argument("--guide-small", action="store_true")
+    ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "

@@ -18,10 +18,6 @@
 #include "../ptlayout.h"
 
 int rk_guide_setup(void *stream);
-int rk_guide_eval(const struct PtArgs *a, const void *grid16, const void *w16,
-                  const float *master, const void *rays, const void *hits,
-                  const unsigned *counts, const void *insts, const void *tris,
-                  float *mix, unsigned grid, void *stream);
 int rk_guide_mix(const struct PtArgs *a, const void *grid16, const void *w16,
                  const float *master, const void *rays, const void *hits,
                  const unsigned *counts, const void *insts, const void *tris,

@@ -207,37 +207,6 @@ struct WaveLds {
   __attribute__((aligned(16))) float out[kWave * GD_OUT];
 };
 
-// Mixture of every queued hit point: mix[i][GD_MIX].
-__global__ __launch_bounds__(64) void k_guide_eval(PtArgs a, Net N,
-                                                   const rray *__restrict__ rays,
-                                                   const rhit *__restrict__ hits,
-                                                   const uint32_t *__restrict__ counts,
-                                                   const dinst *__restrict__ insts,
-                                                   const RTri *__restrict__ tris,
-                                                   float *__restrict__ mix) {
-  __shared__ WaveLds S;
-  const uint32_t n = counts[PT_CNT_CUR];
-  const uint32_t lane = lane_id();
-  for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
-    const uint32_t i = base + lane;
-    float p[3] = {0, 0, 0}, nn[3] = {0, 0, 1};
-    bool ok = false;
-    if (i < n) ok = hit_point(rays[i], hits[i], insts, tris, p, nn);
-    encode(a, N.grid, p, nn, &S.act0[lane * GD_IN]);
-    __syncthreads();
-    forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
-    if (i < n && ok) {
-      float m[GD_MIX];
-      gd_mixture(&S.out[lane * GD_OUT], m);
-      float4 *dst = (float4 *)&mix[(size_t)i * GD_MIX];
-#pragma unroll
-      for (int k = 0; k < GD_MIX / 4; k++)
-        dst[k] = make_float4(m[4 * k], m[4 * k + 1], m[4 * k + 2], m[4 * k + 3]);
-    }
-    __syncthreads();
-  }
-}
-
 // Incident-radiance targets of the frame's records from the final path
 // radiance; accumulates the sum of training weights into wsum.
 __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__restrict__ counts,
@@ -693,16 +662,6 @@ static Net mknet(const void *grid16, const void *w16, const float *master) {
   N.w = (const __bf16 *)w16;
   N.master = master;
   return N;
-}
-
-extern "C" int rk_guide_eval(const PtArgs *a, const void *grid16, const void *w16,
-                             const float *master, const void *rays, const void *hits,
-                             const unsigned *counts, const void *insts, const void *tris,
-                             float *mix, unsigned grid, void *stream) {
-  hipLaunchKernelGGL(k_guide_eval, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
-                     mknet(grid16, w16, master), (const rray *)rays, (const rhit *)hits,
-                     counts, (const dinst *)insts, (const RTri *)tris, mix);
-  return (int)hipGetLastError();
 }
 
 extern "C" int rk_guide_mix(const PtArgs *a, const void *grid16, const void *w16,
