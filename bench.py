@@ -246,6 +246,14 @@ def bench_views(a):
     return 0
 
 
+def rel_mse(x, r):
+    """Relative MSE of an image against a reference: sum (x - r)^2 / sum r^2
+    over pixels and channels."""
+    x = np.asarray(x, np.float64)
+    r = np.asarray(r, np.float64)
+    return float(((x - r) ** 2).sum() / (r ** 2).sum())
+
+
 def bench_guide(a):
     """Equal-error wall time on the occluded-lights scene: plain path tracing
     runs `guide_frames` frames (time T_p, relMSE E_p against a high-spp plain
@@ -254,8 +262,6 @@ def bench_guide(a):
     E_p (time T_g, checked every few frames with the check itself excluded
     from the clock).  Value: T_g / T_p."""
     from nodebleed_amd import REND_MODE_PATHTRACE
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_lib import rel_mse
     L = lib()
     w, h = (W, H) if not a.guide_small else (480, 270)
     spf = a.guide_spf
@@ -319,7 +325,7 @@ def bench_guide(a):
     L.rend_hip_guide_disable(scene.rd)
     emit("guided/plain wall time to equal relMSE", ratio, "ratio", a.guide_frames, 0,
          tp * 1e3 / a.guide_frames,
-         {"workload": f"occluded-lights terrain (1M tris, 8 emitters in chimneys lighting ceiling "
+         {"workload": f"occluded-lights terrain (1M tris, 8 upward emitters capping chimneys light ceiling "
                       f"spots, terrain lit indirectly), {w}x{h}, {spf} spp/frame, max depth "
                       f"{maxdepth}, RR after {rr}; reference {a.guide_ref_frames * spf} spp plain "
                       f"(GPU)",

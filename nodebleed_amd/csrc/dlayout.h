@@ -64,6 +64,9 @@ struct CamArgs {
 #define REND_STACK_MAX      64   /* reference bound, rend.c:834 assert */
 #define REND_LDS_STACK_MIN  8
 #define RK_TRACE_VARIANTS   4
+/* rk_trace_launch flags: replace exactly-zero instance-space direction
+   components by a signed 1e-20 (path-tracing callers only) */
+#define RK_TRACE_NUDGE      1u
 #define REND_TRACE_BLOCK 256
 
 /* Per-launch control block of the traversal kernel: RK_NSEG ray segments,

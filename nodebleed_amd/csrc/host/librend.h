@@ -116,6 +116,7 @@ int rk_trace_launch(const void *arena, unsigned long long tlas,
                     void *spill, unsigned long long *stats, int closest,
                     int count, int rcp_ieee, int variant, unsigned int grid,
                     unsigned int refill, unsigned int max_iter,
-                    const unsigned int *n_dev, unsigned int *diag, void *stream);
+                    const unsigned int *n_dev, unsigned int *diag,
+                    unsigned int flags, void *stream);
 
 #endif

@@ -293,7 +293,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 			unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->rays[cur], t->hits,
 			  NULL, n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 1, 0, G.rcp_ieee, G.variant, cgrid,
-			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, st);
+			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, RK_TRACE_NUDGE, st);
 			if (r)
 				return rl_hipfail(r, "extend trace");
 			if (mix && (r = rl_guide_eval(s, a, t->rays[cur], t->hits, t->counts, st)))
@@ -306,7 +306,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 			slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->srays, NULL, t->occ,
 			  n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 0, 0, G.rcp_ieee, G.variant, agrid, G.refill,
-			  G.max_iter, &t->counts[PT_CNT_SHADOW], G.diag, st);
+			  G.max_iter, &t->counts[PT_CNT_SHADOW], G.diag, RK_TRACE_NUDGE, st);
 			if (r)
 				return rl_hipfail(r, "shadow trace");
 			G.last_slot = slot;

@@ -695,7 +695,7 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
 	  d_occ, n, G.ctl + RK_CTL_WORDS * slot, G.spill, count ? G.stats : NULL,
-	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, G.diag, st);
+	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, G.diag, 0, st);
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;

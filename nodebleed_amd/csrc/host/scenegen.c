@@ -822,10 +822,14 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 	      (unsigned)mter))
 		return -1;
 	if (lights) {
-		/* Occluded lights: 8 small emitters facing up, each inside a
-		 * narrow chimney open at the top, so that no terrain point sees
+		/* Occluded lights: 8 small emitters facing up, each capping a
+		 * narrow chimney above the camera, so that no terrain point sees
 		 * one; they light spots on a diffuse ceiling, and the terrain
-		 * receives light only after a bounce off those spots. */
+		 * receives light only after a bounce off those spots.  The caps
+		 * sit on the chimney tops rather than inside them: an emitter
+		 * inside would light the chimney walls at vanishing distance,
+		 * and paths reaching those walls through the opening would turn
+		 * the 1/d^2 of next-event estimation into fireflies. */
 		int mwall = rs_material(s, 0.7f, 0.7f, 0.7f, 0);
 		int mceil = rs_material(s, 0.8f, 0.8f, 0.8f, 0);
 		int mlit = rs_material(s, 150.0f, 135.0f, 112.0f, RFLAG_EMISSIVE);
@@ -859,8 +863,8 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 				if (rs_quad(s, w0, w1, w2, w3, (unsigned)mwall))
 					return -1;
 			}
-			/* emitter 0.06 below the opening, normal +y */
-			const float lw = 0.02f, ly = 0.34f;
+			/* emitter closing the opening, normal +y */
+			const float lw = hw, ly = y1;
 			float q0[3] = {cx - lw, ly, cz - lw}, q1[3] = {cx - lw, ly, cz + lw};
 			float q2[3] = {cx + lw, ly, cz + lw}, q3[3] = {cx + lw, ly, cz - lw};
 			if (rs_quad(s, q0, q1, q2, q3, (unsigned)mlit))

@@ -64,6 +64,16 @@ __device__ __forceinline__ void ray_setup(RayS &r, float ox, float oy, float oz,
           __builtin_isfinite(r.rx) & __builtin_isfinite(r.ry) & __builtin_isfinite(r.rz);
 }
 
+// An instance-space direction component can cancel to exactly zero (about
+// one in 1e8 products of a random rotation); 1/0 = inf then meets o*inf and
+// the slab distances go NaN, which the x86 rules count as a hit on every box,
+// so the ray walks its whole BLAS.  Path-tracing callers ask for such a
+// component to be replaced by a signed 1e-20 (geometrically the same
+// axis-parallel ray); the reference-exact entry points leave it alone.
+__device__ __forceinline__ float nudge0(float x) {
+  return x == 0.0f ? __builtin_copysignf(1e-20f, x) : x;
+}
+
 // x86 RCPSS from the captured mantissa table (rule proven exhaustively by
 // gen_rcp_table.c).
 // Table entries keep 12 significant bits (the low 11 mantissa bits of every
@@ -160,7 +170,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              uint32_t *__restrict__ ctl, uint2 *__restrict__ spill,
              unsigned long long *__restrict__ stats, uint32_t refill,
              uint32_t max_iter, const uint32_t *__restrict__ n_dev,
-             uint32_t *__restrict__ diag) {
+             uint32_t *__restrict__ diag, uint32_t flags) {
   __shared__ uint16_t s_tab[2048];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
   // stack rows [0, DEPTH) per lane, plus one junk row that branch-free
@@ -395,7 +405,10 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       const float dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
       const float dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
       const float dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
-      ray_setup(cr, ox, oy, oz, dx, dy, dz);
+      if (flags & RK_TRACE_NUDGE)
+        ray_setup(cr, ox, oy, oz, nudge0(dx), nudge0(dy), nudge0(dz));
+      else
+        ray_setup(cr, ox, oy, oz, dx, dy, dz);
       in_blas = true;
       blas_sp = sp;
       base = blas;
@@ -472,13 +485,14 @@ struct LaunchArgs {
   uint32_t refill, max_iter;
   const uint32_t *n_dev;
   uint32_t *diag;
+  uint32_t flags;
 };
 
 template <bool C, bool N, bool I, int D, int W>
 hipError_t launch_one(uint32_t grid, hipStream_t s, const LaunchArgs &a) {
   hipLaunchKernelGGL((k_trace<C, N, I, D, W>), dim3(grid), dim3(REND_TRACE_BLOCK), 0, s,
                      a.arena, a.tlas, a.insts, a.rays, a.hits, a.occ, a.n, a.ctl,
-                     a.spill, a.stats, a.refill, a.max_iter, a.n_dev, a.diag);
+                     a.spill, a.stats, a.refill, a.max_iter, a.n_dev, a.diag, a.flags);
   return hipGetLastError();
 }
 
@@ -552,7 +566,8 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
                                int closest, int count, int rcp_ieee, int variant,
                                unsigned int grid, unsigned int refill,
                                unsigned int max_iter, const unsigned int *n_dev,
-                               unsigned int *diag, void *stream) {
+                               unsigned int *diag, unsigned int flags,
+                               void *stream) {
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_zero_u32, dim3(4), dim3(64), 0, s, ctl, (uint32_t)RK_CTL_ERR);
   hipError_t e = hipGetLastError();
@@ -573,6 +588,7 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
   a.max_iter = max_iter;
   a.n_dev = n_dev;
   a.diag = diag;
+  a.flags = flags;
   const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
   if (count) {
     switch (k) {
