@@ -150,6 +150,11 @@ int          rend_hip_guide_enable(struct rdata *rd, float mix,
 int          rend_hip_guide_disable(struct rdata *rd);
 /* Changes mix and training steps (0 = frozen network) while enabled. */
 int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps);
+/* Training schedule while enabled: records per Adam step (default 32K),
+ * learning rates of the hash grid and of the MLP, and whether records are
+ * drawn in proportion to their target (importance != 0) or uniformly. */
+int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
+                                     float lr_grid, float lr_mlp, int importance);
 /* out: last training loss, records of the last batch, Adam steps, enabled. */
 int          rend_hip_guide_info(struct rdata *rd, float out[4]);
 /* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),
@@ -161,7 +166,8 @@ int          rend_hip_guide_forward_host(struct rdata *rd, const float *pn,
 int          rend_hip_guide_grad_host(struct rdata *rd, const void *records,
                                       unsigned int n, float *grad, float *loss);
 /* `steps` Adam steps on n host training records (GdRecord layout, target
- * set), 32K records sampled per step; loss = last step's loss. */
+ * set), sampled per step as rend_hip_guide_training sets; loss = last
+ * step's loss. */
 int          rend_hip_guide_train_host(struct rdata *rd, const void *records,
                                        unsigned int n, unsigned int steps,
                                        float *loss);
@@ -169,6 +175,11 @@ int          rend_hip_guide_train_host(struct rdata *rd, const void *records,
  * positions+normals: per lobe weight, mean direction (3), concentration. */
 int          rend_hip_guide_mix_host(struct rdata *rd, const float *pn,
                                      unsigned int n, float *out);
+/* Inclusive prefix sums of n non-negative floats by the record-sampling
+ * kernels, in fixed point: floor(v * 2^(60 - e)) summed exactly, with e the
+ * exponent for which 2^(e-1) <= wsum < 2^e. */
+int          rend_hip_guide_cdf_host(const float *values, unsigned int n, float wsum,
+                                     unsigned long long *cdf);
 /* D = A (16x32) * B (32x16), row-major fp32 in, bf16 MFMA inside. */
 int          rend_hip_mfma_check(const float *A, const float *B, float *D);
 

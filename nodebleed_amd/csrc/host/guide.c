@@ -24,12 +24,16 @@ int rk_guide_mix(const struct PtArgs *a, const void *grid16, const void *w16,
                  const float *pn, unsigned npn, float *mix, unsigned grid,
                  void *stream);
 int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
-                     float *wsum, unsigned cap, void *stream);
+                     float *wsum, float *dense, unsigned cap, void *stream);
+unsigned rk_guide_cdf_tiles(unsigned cap);
+int rk_guide_cdf(const float *dense, const float *wsum, unsigned long long *tsum,
+                 unsigned long long *cdf, const unsigned *counts, unsigned cap,
+                 void *stream);
 int rk_guide_train(const struct PtArgs *a, const void *grid16, const void *w16,
                    const float *master, const void *rec, const unsigned *counts,
                    unsigned cap, unsigned first, unsigned chunk,
-                   const float *wsum, float *grad, float *loss, unsigned grid,
-                   void *stream);
+                   const float *wsum, const unsigned long long *cdf, float *grad,
+                   float *loss, unsigned grid, void *stream);
 int rk_guide_adam(float *master, float *grad, float *m1, float *m2,
                   void *grid16, void *w16, unsigned step, float lr_grid,
                   float lr_mlp, void *stream);
@@ -60,6 +64,9 @@ void rl_guide_release(struct rl_scene *s)
 	dfree(&g->w16);
 	dfree((void **)&g->mix);
 	dfree(&g->rec);
+	dfree((void **)&g->dense);
+	dfree((void **)&g->cdf);
+	dfree((void **)&g->tsum);
 	dfree((void **)&g->scal);
 	free(g);
 	s->guide = NULL;
@@ -185,6 +192,8 @@ int rend_hip_guide_enable(struct rdata *rd, float mix, unsigned int train_stride
 	g->steps = steps;
 	g->lr_grid = lr > 0.0f ? lr : 1e-2f;
 	g->lr_mlp = g->lr_grid * 0.1f;
+	g->chunk = GD_DEFAULT_CHUNK;
+	g->importance = GD_DEFAULT_IMPORTANCE;
 	g->enabled = 1;
 	hipError_t e = hipStreamSynchronize(G.stream);
 	return e ? rl_hipfail(e, "guide enable") : 0;
@@ -207,6 +216,25 @@ int rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps)
 		return rl_fail(REND_EINVAL, "guide mix %f", (double)mix);
 	s->guide->mix_prob = mix;
 	s->guide->steps = steps;
+	return 0;
+}
+
+int rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
+                            float lr_grid, float lr_mlp, int importance)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	if (records_per_step < 64 || records_per_step > (1u << 22))
+		return rl_fail(REND_EINVAL, "records per step %u (64..4M)", records_per_step);
+	if (!(lr_grid > 0.0f) || !(lr_mlp > 0.0f) || lr_grid > 1.0f || lr_mlp > 1.0f)
+		return rl_fail(REND_EINVAL, "learning rates %g %g", (double)lr_grid,
+		  (double)lr_mlp);
+	struct rl_guide *g = s->guide;
+	g->chunk = records_per_step;
+	g->lr_grid = lr_grid;
+	g->lr_mlp = lr_mlp;
+	g->importance = importance != 0;
 	return 0;
 }
 
@@ -260,6 +288,18 @@ int rl_guide_buffers(struct rl_scene *s, size_t cap)
 			return rl_hipfail(e, "guide record buffer");
 		g->rec_cap = rc;
 	}
+	if (rc > g->cdf_cap) {
+		dfree((void **)&g->dense);
+		dfree((void **)&g->cdf);
+		dfree((void **)&g->tsum);
+		g->cdf_cap = 0;
+		if ((e = hipMalloc((void **)&g->dense, rc * sizeof(float))) ||
+		    (e = hipMalloc((void **)&g->cdf, rc * sizeof(unsigned long long))) ||
+		    (e = hipMalloc((void **)&g->tsum,
+		      (rk_guide_cdf_tiles((unsigned)rc) + 1) * sizeof(unsigned long long))))
+			return rl_hipfail(e, "guide record cdf");
+		g->cdf_cap = rc;
+	}
 	return 0;
 }
 
@@ -274,7 +314,8 @@ int rl_guide_eval(struct rl_scene *s, const struct PtArgs *a, const void *rays,
 }
 
 /* After a batch: targets from the final path radiance, then training
- * steps of 32K records each, each followed by an Adam update. */
+ * steps of g->chunk records each (drawn in proportion to their targets, or
+ * uniformly), each followed by an Adam update. */
 int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths,
                    unsigned *counts, hipStream_t st)
 {
@@ -285,14 +326,18 @@ int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths
 	unsigned rc = a->train_cap;
 	if ((r = rk_fill_f32(g->scal, 2, 0.0f, st)))
 		return rl_hipfail(r, "guide scalars");
-	if ((r = rk_guide_targets(g->rec, counts, paths, g->scal, rc, st)))
+	unsigned long long *cdf = g->importance ? g->cdf : NULL;
+	if ((r = rk_guide_targets(g->rec, counts, paths, g->scal, cdf ? g->dense : NULL, rc,
+	  st)))
 		return rl_hipfail(r, "guide targets");
-	const unsigned chunk = 32768;
+	if (cdf && (r = rk_guide_cdf(g->dense, g->scal, g->tsum, cdf, counts, rc, st)))
+		return rl_hipfail(r, "guide record cdf");
 	for (unsigned k = 0; k < g->steps; k++) {
 		/* seed of the step's record sample */
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(a, g->grid16, g->w16, g->master, g->rec, counts, rc,
-		  first, chunk, g->scal, g->grad, g->scal + 1, (unsigned)G.cus * 4u, st)))
+		  first, g->chunk, g->scal, cdf, g->grad, g->scal + 1, (unsigned)G.cus * 4u,
+		  st)))
 			return rl_hipfail(r, "guide train");
 		g->step++;
 		if ((r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16,
@@ -425,7 +470,7 @@ int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int
 		goto out;
 	}
 	if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, NULL, n, 0, n, dsc,
-	  g->grad, dsc + 1, (unsigned)((n + 63) / 64), G.stream))) {
+	  NULL, g->grad, dsc + 1, (unsigned)((n + 63) / 64), G.stream))) {
 		r = rl_hipfail(r, "guide grad");
 		goto out;
 	}
@@ -442,8 +487,9 @@ out:
 	return r;
 }
 
-/* `steps` training steps (32K records sampled per step, then Adam) on n
- * host records with `target` set; loss receives the last step's loss. */
+/* `steps` training steps on n host records with `target` set, as a frame's
+ * training runs them (g->chunk records per step, drawn in proportion to
+ * target or uniformly, then Adam); loss receives the last step's loss. */
 int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned int n,
                               unsigned int steps, float *loss)
 {
@@ -456,21 +502,42 @@ int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned in
 	struct PtArgs a;
 	test_args(g, &a);
 	void *drec = NULL;
-	float *dsc = NULL;
+	float *dsc = NULL, *dd = NULL, *hd = NULL;
+	unsigned long long *dcdf = NULL, *dts = NULL;
+	unsigned *dcnt = NULL;
 	hipError_t e;
 	int r = 0;
 	float sc[2] = {0.0f, 0.0f};
+	unsigned cnt[PT_NCOUNTS] = {0};
+	cnt[PT_CNT_TRAIN] = n;
 	const struct GdRecord *rr = records;
-	for (unsigned i = 0; i < n; i++)
+	if (!(hd = malloc((size_t)n * sizeof(float))))
+		return rl_fail(REND_ENOMEM, "guide train");
+	for (unsigned i = 0; i < n; i++) {
 		sc[0] += rr[i].target;
+		hd[i] = rr[i].target;
+	}
 	if ((e = hipMalloc(&drec, (size_t)n * sizeof(struct GdRecord) + 4)) ||
 	    (e = hipMalloc((void **)&dsc, 2 * sizeof(float))) ||
+	    (e = hipMalloc((void **)&dcnt, sizeof(cnt))) ||
+	    (e = hipMalloc((void **)&dd, (size_t)n * sizeof(float))) ||
+	    (e = hipMalloc((void **)&dcdf, (size_t)n * sizeof(unsigned long long))) ||
+	    (e = hipMalloc((void **)&dts,
+	      (rk_guide_cdf_tiles(n) + 1) * sizeof(unsigned long long))) ||
 	    (e = hipMemcpy(drec, records, (size_t)n * sizeof(struct GdRecord),
-	      hipMemcpyHostToDevice))) {
+	      hipMemcpyHostToDevice)) ||
+	    (e = hipMemcpy(dcnt, cnt, sizeof(cnt), hipMemcpyHostToDevice)) ||
+	    (e = hipMemcpy(dd, hd, (size_t)n * sizeof(float), hipMemcpyHostToDevice)) ||
+	    (e = hipMemcpy(dsc, sc, sizeof(sc), hipMemcpyHostToDevice))) {
 		r = rl_hipfail(e, "guide train buffers");
 		goto out;
 	}
-	const unsigned chunk = n < 32768u ? n : 32768u;
+	unsigned long long *cdf = g->importance ? dcdf : NULL;
+	if (cdf && (r = rk_guide_cdf(dd, dsc, dts, cdf, dcnt, n, G.stream))) {
+		r = rl_hipfail(r, "guide record cdf");
+		goto out;
+	}
+	const unsigned chunk = n < g->chunk ? n : g->chunk;
 	for (unsigned k = 0; k < steps; k++) {
 		sc[1] = 0.0f;
 		if ((e = hipMemcpyAsync(dsc, sc, sizeof(sc), hipMemcpyHostToDevice, G.stream))) {
@@ -478,8 +545,8 @@ int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned in
 			goto out;
 		}
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
-		if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, NULL, n, first,
-		  chunk, dsc, g->grad, dsc + 1, (unsigned)G.cus * 4u, G.stream))) {
+		if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, dcnt, n, first,
+		  chunk, dsc, cdf, g->grad, dsc + 1, (unsigned)G.cus * 4u, G.stream))) {
 			r = rl_hipfail(r, "guide train");
 			goto out;
 		}
@@ -496,8 +563,58 @@ int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned in
 	if (loss)
 		*loss = sc[1];
 out:
+	free(hd);
 	dfree(&drec);
 	dfree((void **)&dsc);
+	dfree((void **)&dcnt);
+	dfree((void **)&dd);
+	dfree((void **)&dcdf);
+	dfree((void **)&dts);
+	return r;
+}
+
+/* Fixed-point prefix sums of n host values (scaled by 2^(60 - e), 2^e
+ * above wsum) by the training-record cdf kernels. */
+int rend_hip_guide_cdf_host(const float *values, unsigned int n, float wsum,
+                            unsigned long long *cdf)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	if (!n)
+		return 0;
+	float *dv = NULL, *dw = NULL;
+	unsigned long long *dts = NULL, *dc = NULL;
+	unsigned *dcnt = NULL;
+	unsigned cnt[PT_NCOUNTS] = {0};
+	cnt[PT_CNT_TRAIN] = n;
+	hipError_t e;
+	if ((e = hipMalloc((void **)&dv, (size_t)n * sizeof(float))) ||
+	    (e = hipMalloc((void **)&dw, sizeof(float))) ||
+	    (e = hipMalloc((void **)&dc, (size_t)n * sizeof(unsigned long long))) ||
+	    (e = hipMalloc((void **)&dts,
+	      (rk_guide_cdf_tiles(n) + 1) * sizeof(unsigned long long))) ||
+	    (e = hipMalloc((void **)&dcnt, sizeof(cnt))) ||
+	    (e = hipMemcpy(dv, values, (size_t)n * sizeof(float), hipMemcpyHostToDevice)) ||
+	    (e = hipMemcpy(dw, &wsum, sizeof(float), hipMemcpyHostToDevice)) ||
+	    (e = hipMemcpy(dcnt, cnt, sizeof(cnt), hipMemcpyHostToDevice))) {
+		r = rl_hipfail(e, "cdf buffers");
+		goto out;
+	}
+	if ((r = rk_guide_cdf(dv, dw, dts, dc, dcnt, n, G.stream))) {
+		r = rl_hipfail(r, "cdf");
+		goto out;
+	}
+	if ((e = hipStreamSynchronize(G.stream)) ||
+	    (e = hipMemcpy(cdf, dc, (size_t)n * sizeof(unsigned long long),
+	      hipMemcpyDeviceToHost)))
+		r = rl_hipfail(e, "cdf");
+out:
+	dfree((void **)&dv);
+	dfree((void **)&dw);
+	dfree((void **)&dc);
+	dfree((void **)&dts);
+	dfree((void **)&dcnt);
 	return r;
 }
 

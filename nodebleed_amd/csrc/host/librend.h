@@ -44,8 +44,13 @@ struct rl_guide {
 	size_t    cap;                      /* paths the mixture buffer holds */
 	size_t    rec_cap;                  /* records the record buffer holds */
 	float     gmin[3], ginv[3];
+	float    *dense;                    /* record targets, dense */
+	unsigned long long *cdf, *tsum;     /* fixed-point target prefix sums, tile sums */
+	size_t    cdf_cap;
 	float     mix_prob, lr_grid, lr_mlp;
 	unsigned  stride, steps, step, last_records;
+	unsigned  chunk;                    /* records per training step */
+	int       importance;               /* draw records in proportion to target */
 	int       enabled;
 };
 

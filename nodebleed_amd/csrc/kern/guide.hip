@@ -211,7 +211,7 @@ struct WaveLds {
 // radiance; accumulates the sum of training weights into wsum.
 __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__restrict__ counts,
                                 const PtPath *__restrict__ paths, float *__restrict__ wsum,
-                                uint32_t cap) {
+                                float *__restrict__ dense, uint32_t cap) {
   const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
   float acc = 0.0f;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -222,10 +222,115 @@ __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__re
     if (!(li < 1e6f)) li = 0.0f;
     const float c = fmaxf(r.nrm[0] * r.dir[0] + r.nrm[1] * r.dir[1] + r.nrm[2] * r.dir[2], 0.0f);
     r.target = r.pdf > 0.0f ? li * c / r.pdf : 0.0f;
+    if (!(r.target < 1e30f)) r.target = 0.0f;
+    if (dense) dense[i] = r.target;
     acc += r.target;
   }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
   if ((threadIdx.x & 63u) == 0 && acc > 0.0f) atomicAdd(wsum, acc);
+}
+
+// Records drawn in proportion to their targets: inclusive prefix sums of the
+// targets in 64-bit fixed point (target * 2^(60 - e) with 2^e above the
+// batch's target sum, so sums stay below 2^62 and are exact in any order,
+// hence monotone), in three passes: sums of 4096-record tiles, one block
+// scanning the tile sums, then each tile scanned with its offset.  A training
+// step finds record j for u in [cdf[j-1], cdf[j]) by binary search.
+constexpr uint32_t kScanPer = 16;                   // records per thread
+constexpr uint32_t kScanTile = 256 * kScanPer;      // records per 256-thread block
+
+__device__ __forceinline__ float cdf_scale(float wsum) {
+  int e;
+  frexpf(fmaxf(wsum, 1e-30f), &e);  // wsum < 2^e
+  return ldexpf(1.0f, 60 - e);
+}
+
+__device__ __forceinline__ uint64_t cdf_quant(float t, float scale) {
+  return t > 0.0f ? (uint64_t)(t * scale) : 0ull;
+}
+
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int o) {
+  const uint32_t lo = __shfl_up((uint32_t)v, o), hi = __shfl_up((uint32_t)(v >> 32), o);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Exclusive scan over a 256-thread block; total receives the block's sum.
+__device__ __forceinline__ uint64_t block_excl_scan256(uint64_t v, uint64_t *s_w,
+                                                       uint64_t &total) {
+  const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  uint64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = shfl_up64(x, o);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  uint64_t before = 0;
+  for (uint32_t k = 0; k < w; k++) before += s_w[k];
+  total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+  __syncthreads();
+  return before + x - v;
+}
+
+__global__ __launch_bounds__(256) void k_guide_tile_sums(const float *__restrict__ dense,
+                                                         const uint32_t *__restrict__ counts,
+                                                         uint32_t cap,
+                                                         const float *__restrict__ wsum,
+                                                         uint64_t *__restrict__ tsum) {
+  __shared__ uint64_t s_w[4];
+  const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
+  const uint32_t base = blockIdx.x * kScanTile;
+  if (base >= n) return;
+  const float sc = cdf_scale(wsum[0]);
+  uint64_t v = 0;
+  for (uint32_t k = threadIdx.x; k < kScanTile; k += 256) {
+    const uint32_t i = base + k;
+    if (i < n) v += cdf_quant(dense[i], sc);
+  }
+  uint64_t total;
+  block_excl_scan256(v, s_w, total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_guide_scan_sums(uint64_t *__restrict__ tsum,
+                                                         const uint32_t *__restrict__ counts,
+                                                         uint32_t cap) {
+  __shared__ uint64_t s_w[4];
+  const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
+  const uint32_t nt = (n + kScanTile - 1) / kScanTile;
+  uint64_t carry = 0;
+  for (uint32_t b = 0; b < nt; b += 256) {
+    const uint32_t i = b + threadIdx.x;
+    const uint64_t v = i < nt ? tsum[i] : 0ull;
+    uint64_t total;
+    const uint64_t ex = block_excl_scan256(v, s_w, total);
+    if (i < nt) tsum[i] = carry + ex;  // exclusive offset of tile i
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_guide_tile_scan(const float *__restrict__ dense,
+                                                         const uint64_t *__restrict__ tsum,
+                                                         const uint32_t *__restrict__ counts,
+                                                         uint32_t cap,
+                                                         const float *__restrict__ wsum,
+                                                         uint64_t *__restrict__ cdf) {
+  __shared__ uint64_t s_w[4];
+  const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
+  const uint32_t base = blockIdx.x * kScanTile;
+  if (base >= n) return;
+  const float sc = cdf_scale(wsum[0]);
+  const uint32_t i0 = base + threadIdx.x * kScanPer;
+  uint64_t run = 0;
+  for (uint32_t k = 0; k < kScanPer; k++)
+    if (i0 + k < n) run += cdf_quant(dense[i0 + k], sc);
+  uint64_t total;
+  uint64_t acc = block_excl_scan256(run, s_w, total) + tsum[blockIdx.x];
+  for (uint32_t k = 0; k < kScanPer; k++)
+    if (i0 + k < n) {
+      acc += cdf_quant(dense[i0 + k], sc);
+      cdf[i0 + k] = acc;
+    }
 }
 
 // Forward + backward over `chunk` records drawn from the batch: gradients of
@@ -236,6 +341,7 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
                                                     uint32_t cap, uint32_t first,
                                                     uint32_t chunk,
                                                     const float *__restrict__ wsum,
+                                                    const uint64_t *__restrict__ cdf,
                                                     float *__restrict__ grad,
                                                     float *__restrict__ loss) {
   __shared__ WaveLds S;
@@ -250,13 +356,34 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
   const uint32_t n = min(chunk, total);
   const float wmean = wsum[0] / (float)(total ? total : 1u);
   const float inv = wmean > 0.0f ? 1.0f / (wmean * (float)(n ? n : 1u)) : 0.0f;
+  // With a cdf: systematic sampling of n records with probability
+  // target / sum (offset (first >> 8) / 2^24 within each of the n strata),
+  // each then weighted 1/n -- the same expected gradient with far less
+  // variance when a few records carry most of the radiance.
+  const uint64_t ctot = (cdf && total) ? cdf[total - 1] : 0ull;
+  const double cstep = (double)ctot / (double)(n ? n : 1u);
+  const double xi = (double)(first >> 8) * (1.0 / 16777216.0);
   float lacc = 0.0f;
   for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
     const uint32_t i = base + lane;
-    const bool valid = i < n;
+    bool valid = i < n;
     GdRecord r;
-    if (valid) r = rec[((uint64_t)i * 2654435761ull + first) % total];
-    else {
+    float wgt = 0.0f;
+    if (valid && cdf) {
+      const uint64_t u = (uint64_t)(((double)i + xi) * cstep);
+      uint32_t lo = 0, hi = total;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1; else hi = mid;
+      }
+      r = rec[lo < total ? lo : total - 1];
+      valid = ctot > 0 && r.target > 0.0f;
+      wgt = valid ? 1.0f / (float)n : 0.0f;
+    } else if (valid) {
+      r = rec[((uint64_t)i * 2654435761ull + first) % total];
+      wgt = r.target * inv;
+    }
+    if (!valid) {
       for (int k = 0; k < 3; k++) { r.pos[k] = 0.0f; r.nrm[k] = 0.0f; r.dir[k] = 0.0f; }
       r.nrm[2] = 1.0f; r.dir[2] = 1.0f; r.target = 0.0f;
     }
@@ -265,7 +392,6 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
     forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
     // d loss / d raw outputs for this lane's sample
     float g[GD_OUT];
-    const float wgt = valid ? r.target * inv : 0.0f;
     lacc += gd_loss_grad(&S.out[lane * GD_OUT], r.dir, wgt, g);
     for (int k = 0; k < GD_OUT; k++) dout[lane * GD_OUT + k] = (__bf16)g[k];
     __syncthreads();
@@ -677,20 +803,42 @@ extern "C" int rk_guide_mix(const PtArgs *a, const void *grid16, const void *w16
 }
 
 extern "C" int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
-                                float *wsum, unsigned cap, void *stream) {
+                                float *wsum, float *dense, unsigned cap, void *stream) {
   hipLaunchKernelGGL(k_guide_targets, dim3(256), dim3(256), 0, (hipStream_t)stream,
-                     (GdRecord *)rec, counts, (const PtPath *)paths, wsum, cap);
+                     (GdRecord *)rec, counts, (const PtPath *)paths, wsum, dense, cap);
+  return (int)hipGetLastError();
+}
+
+extern "C" unsigned rk_guide_cdf_tiles(unsigned cap) {
+  return (cap + kScanTile - 1) / kScanTile;
+}
+
+// Fixed-point inclusive prefix sums cdf[i] of dense[0, min(counts[train],
+// cap)) scaled by the sum wsum[0]; tsum holds rk_guide_cdf_tiles(cap) words.
+extern "C" int rk_guide_cdf(const float *dense, const float *wsum, unsigned long long *tsum,
+                            unsigned long long *cdf, const unsigned *counts, unsigned cap,
+                            void *stream) {
+  const unsigned nt = (cap + kScanTile - 1) / kScanTile;
+  if (!nt) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_guide_tile_sums, dim3(nt), dim3(256), 0, st, dense, counts, cap, wsum,
+                     (uint64_t *)tsum);
+  hipLaunchKernelGGL(k_guide_scan_sums, dim3(1), dim3(256), 0, st, (uint64_t *)tsum, counts,
+                     cap);
+  hipLaunchKernelGGL(k_guide_tile_scan, dim3(nt), dim3(256), 0, st, dense,
+                     (const uint64_t *)tsum, counts, cap, wsum, (uint64_t *)cdf);
   return (int)hipGetLastError();
 }
 
 extern "C" int rk_guide_train(const PtArgs *a, const void *grid16, const void *w16,
                               const float *master, const void *rec, const unsigned *counts,
                               unsigned cap, unsigned first, unsigned chunk, const float *wsum,
-                              float *grad, float *loss, unsigned grid, void *stream) {
+                              const unsigned long long *cdf, float *grad, float *loss,
+                              unsigned grid, void *stream) {
   if (!chunk) return 0;
   hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
                      mknet(grid16, w16, master), (const GdRecord *)rec, counts, cap, first,
-                     chunk, wsum, grad, loss);
+                     chunk, wsum, (const uint64_t *)cdf, grad, loss);
   return (int)hipGetLastError();
 }
 

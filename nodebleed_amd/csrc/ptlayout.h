@@ -63,6 +63,10 @@ struct PtLight {
 #define GD_HID     64
 #define GD_OUT     64
 #define GD_LOBES   8
+/* training records per optimiser step unless set otherwise */
+#define GD_DEFAULT_CHUNK 32768u
+/* records drawn in proportion to their target (1) or uniformly (0) */
+#define GD_DEFAULT_IMPORTANCE 1
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 

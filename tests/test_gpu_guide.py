@@ -241,3 +241,67 @@ def test_training_learns_a_known_lobe(guided_scene):
         f"{np.median(p_opp):.4f}, uniform {1 / (4 * np.pi):.4f}")
     # a vMF with kappa 40 has density 6.4 at its mean; uniform 0.08
     assert np.median(p_mu) > 3.0 and np.median(p_opp) < 0.05
+
+
+def test_record_cdf_matches_cumsum():
+    """Fixed-point prefix sums of training-record targets (tile sums, scan of
+    the tile sums, tile scans) equal an exact integer cumulative sum of
+    floor(v * 2^(60 - e)), across tile boundaries and with zero runs; the
+    scale keeps the total below 2^62."""
+    L = lib()
+    rng = np.random.default_rng(4)
+    for n in (1, 100, 4095, 4096, 4097, 70_001, (1 << 20) + 3):
+        v = rng.exponential(1.0, n).astype(np.float32)
+        v[rng.random(n) < 0.3] = 0.0
+        v[0] = 0.0
+        wsum = np.float32(v.astype(np.float64).sum())
+        say(f"record cdf n={n}")
+        out = np.zeros(n, np.uint64)
+        check(L.rend_hip_guide_cdf_host(ptr(v), n, wsum, ptr(out)), "cdf")
+        e = np.frexp(max(float(wsum), 1e-30))[1]
+        q = np.floor(v.astype(np.float64) * 2.0 ** (60 - e)).astype(np.uint64)
+        want = np.cumsum(q, dtype=np.uint64)
+        assert np.array_equal(out, want), n
+        assert int(want[-1]) < 1 << 62
+
+
+def test_importance_sampled_training_learns_a_known_lobe(guided_scene):
+    """Records with uniformly random directions whose targets follow one vMF
+    lobe (kappa 40): drawing records in proportion to target must teach the
+    mixture that lobe, as the uniformly drawn, target-weighted steps do."""
+    s = guided_scene
+    L = lib()
+    check(L.rend_hip_guide_enable(s.rd, 0.5, 2, 2, 1e-2, 22), "guide re-init")
+    check(L.rend_hip_guide_training(s.rd, 32768, 1e-2, 1e-2, 1), "schedule")
+    rng = np.random.default_rng(9)
+    ab = s.aabbs()
+    lo, hi = ab["min"].min(axis=0), ab["max"].max(axis=0)
+    c = lo + (hi - lo) * np.array([0.7, 0.4, 0.3])
+    mu = np.array([0.0, 0.6, -0.8])
+    k = 40.0
+    loss = C.c_float(0)
+    say("guide training, records drawn by target: 150 steps")
+    for _ in range(150):
+        n = 65536
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rec = np.zeros(n, G.RECORD)
+        rec["pos"] = c + rng.uniform(-0.1, 0.1, (n, 3)) * (hi - lo)
+        rec["nrm"], rec["dir"], rec["pdf"] = (0, 1, 0), d, 1 / (4 * np.pi)
+        rec["target"] = np.exp(k * (d @ mu - 1)) * 4 * np.pi
+        check(L.rend_hip_guide_train_host(s.rd, ptr(rec), n, 1, C.byref(loss)))
+    pos = c + rng.uniform(-0.1, 0.1, (256, 3)) * (hi - lo)
+    pn = np.ascontiguousarray(np.concatenate([pos, np.tile([0, 1.0, 0], (256, 1))], 1),
+                              np.float32)
+    mix = np.zeros((256, G.LOBES, 5), np.float32)
+    check(L.rend_hip_guide_mix_host(s.rd, ptr(pn), 256, ptr(mix)))
+    pi, m, kap = mix[..., 0], mix[..., 1:4], mix[..., 4]
+
+    def density(dirn):
+        cth = (m * dirn).sum(axis=2)
+        return (pi * kap / (2 * np.pi * -np.expm1(-2 * kap)) * np.exp(kap * (cth - 1))).sum(axis=1)
+
+    p_mu, p_opp = density(mu), density(-mu)
+    say(f"loss {loss.value:.3f}; density at the lobe {np.median(p_mu):.2f}, opposite "
+        f"{np.median(p_opp):.4f}")
+    assert np.median(p_mu) > 3.0 and np.median(p_opp) < 0.05
