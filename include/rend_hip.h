@@ -141,9 +141,11 @@ int          rend_hip_pt_graphs(int on);
  * A hash-grid-encoded 3x64 MLP outputs an 8-lobe vMF mixture per shading
  * point; the indirect direction is drawn from mix * mixture + (1 - mix) *
  * cosine BSDF (one-sample MIS, unbiased for 0 <= mix < 1).  The network is
- * trained online from the vertices of every train_stride-th path, `steps`
- * Adam steps of 32K records per wavefront batch, learning rate lr (hash
- * grid; MLP lr / 10).  The proposal is used from the first trained batch on. */
+ * trained online from the vertices of every train_stride-th path: `steps`
+ * Adam steps of 32K records per frame, spread over its wavefront batches,
+ * records drawn in proportion to their radiance targets, learning rate lr
+ * for the hash grid and the MLP.  The proposal is used from the first
+ * trained batch on. */
 int          rend_hip_guide_enable(struct rdata *rd, float mix,
                                    unsigned int train_stride, unsigned int steps,
                                    float lr, unsigned int seed);
@@ -155,6 +157,10 @@ int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps)
  * drawn in proportion to their target (importance != 0) or uniformly. */
 int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
                                      float lr_grid, float lr_mlp, int importance);
+/* Which vertices are guided: those at depth < depth (default 2: the first
+ * two bounces); share != 0 evaluates the depth-0 proposal once per pixel of
+ * a single view and uses it for all of the pixel's samples (default on). */
+int          rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share);
 /* out: last training loss, records of the last batch, Adam steps, enabled. */
 int          rend_hip_guide_info(struct rdata *rd, float out[4]);
 /* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),

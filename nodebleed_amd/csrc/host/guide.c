@@ -21,8 +21,8 @@ int rk_guide_setup(void *stream);
 int rk_guide_mix(const struct PtArgs *a, const void *grid16, const void *w16,
                  const float *master, const void *rays, const void *hits,
                  const unsigned *counts, const void *insts, const void *tris,
-                 const float *pn, unsigned npn, float *mix, unsigned grid,
-                 void *stream);
+                 const float *pn, unsigned npn, unsigned share, float *mix,
+                 unsigned grid, void *stream);
 int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
                      float *wsum, float *dense, unsigned cap, void *stream);
 unsigned rk_guide_cdf_tiles(unsigned cap);
@@ -110,7 +110,7 @@ static int upload_params(struct rl_guide *g, const float *p, hipStream_t st)
 	if (hipMemsetAsync(g->m1, 0, GD_NPARAM * sizeof(float), st) ||
 	    hipMemsetAsync(g->m2, 0, GD_NPARAM * sizeof(float), st) ||
 	    hipMemsetAsync(g->grad, 0, GD_NPARAM * sizeof(float), st) ||
-	    hipMemsetAsync(g->w16, 0, 2 * GD_NW16 * 2, st))
+	    hipMemsetAsync(g->w16, 0, GD_W16_ELEMS * 2, st))
 		return rl_fail(REND_ENODEV, "guide state clear");
 	int r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16, 1,
 	  0.0f, 0.0f, st);
@@ -133,7 +133,7 @@ static struct rl_guide *guide_alloc(struct rl_scene *s)
 	    (e = hipMalloc((void **)&g->m1, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc((void **)&g->m2, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc(&g->grid16, GD_N_GRID * 2)) ||
-	    (e = hipMalloc(&g->w16, 2 * GD_NW16 * 2)) ||
+	    (e = hipMalloc(&g->w16, GD_W16_ELEMS * 2)) ||
 	    (e = hipMalloc((void **)&g->scal, 8 * sizeof(float)))) {
 		s->guide = g;
 		rl_guide_release(s);
@@ -191,9 +191,11 @@ int rend_hip_guide_enable(struct rdata *rd, float mix, unsigned int train_stride
 	g->stride = train_stride ? train_stride : 4;
 	g->steps = steps;
 	g->lr_grid = lr > 0.0f ? lr : 1e-2f;
-	g->lr_mlp = g->lr_grid * 0.1f;
+	g->lr_mlp = g->lr_grid;
 	g->chunk = GD_DEFAULT_CHUNK;
 	g->importance = GD_DEFAULT_IMPORTANCE;
+	g->depth = GD_DEFAULT_DEPTH;
+	g->share = 1;
 	g->enabled = 1;
 	hipError_t e = hipStreamSynchronize(G.stream);
 	return e ? rl_hipfail(e, "guide enable") : 0;
@@ -238,6 +240,18 @@ int rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
 	return 0;
 }
 
+int rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	if (depth < 1)
+		return rl_fail(REND_EINVAL, "guided depth %u (>= 1)", depth);
+	s->guide->depth = depth;
+	s->guide->share = share != 0;
+	return 0;
+}
+
 int rend_hip_guide_info(struct rdata *rd, float out[4])
 {
 	struct rl_scene *s = rl_scene(rd);
@@ -265,6 +279,8 @@ void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 	memcpy(a->ginv, g->ginv, sizeof(a->ginv));
 	a->train_stride = g->steps ? g->stride : 0;
 	a->train_cap = (unsigned)(cap / g->stride + 64);
+	a->guide_depth = g->depth;
+	a->guide_share = 1;
 }
 
 /* Per-batch buffers: mixtures per queued path and training records (as
@@ -305,22 +321,22 @@ int rl_guide_buffers(struct rl_scene *s, size_t cap)
 
 /* Mixture evaluation for the current queue (before shading). */
 int rl_guide_eval(struct rl_scene *s, const struct PtArgs *a, const void *rays,
-                  const void *hits, const unsigned *counts, hipStream_t st)
+                  const void *hits, const unsigned *counts, unsigned share, hipStream_t st)
 {
 	struct rl_guide *g = s->guide;
 	int r = rk_guide_mix(a, g->grid16, g->w16, g->master, rays, hits, counts,
-	  s->insts, s->tris, NULL, 0, g->mix, (unsigned)G.cus * 4u, st);
+	  s->insts, s->tris, NULL, 0, share, g->mix, (unsigned)G.cus * 4u, st);
 	return r ? rl_hipfail(r, "guide eval") : 0;
 }
 
-/* After a batch: targets from the final path radiance, then training
- * steps of g->chunk records each (drawn in proportion to their targets, or
- * uniformly), each followed by an Adam update. */
+/* After a batch: targets from the final path radiance, then `steps`
+ * training steps of g->chunk records each (drawn in proportion to their
+ * targets, or uniformly), each followed by an Adam update. */
 int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths,
-                   unsigned *counts, hipStream_t st)
+                   unsigned *counts, unsigned steps, hipStream_t st)
 {
 	struct rl_guide *g = s->guide;
-	if (!g->steps)
+	if (!steps)
 		return 0;
 	int r;
 	unsigned rc = a->train_cap;
@@ -332,11 +348,11 @@ int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths
 		return rl_hipfail(r, "guide targets");
 	if (cdf && (r = rk_guide_cdf(g->dense, g->scal, g->tsum, cdf, counts, rc, st)))
 		return rl_hipfail(r, "guide record cdf");
-	for (unsigned k = 0; k < g->steps; k++) {
+	for (unsigned k = 0; k < steps; k++) {
 		/* seed of the step's record sample */
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(a, g->grid16, g->w16, g->master, g->rec, counts, rc,
-		  first, g->chunk, g->scal, cdf, g->grad, g->scal + 1, (unsigned)G.cus * 4u,
+		  first, g->chunk, g->scal, cdf, g->grad, g->scal + 1, (unsigned)G.cus * 2u,
 		  st)))
 			return rl_hipfail(r, "guide train");
 		g->step++;
@@ -428,7 +444,7 @@ int rend_hip_guide_mix_host(struct rdata *rd, const float *pn, unsigned int n, f
 		goto out;
 	}
 	if ((r = rk_guide_mix(&a, g->grid16, g->w16, g->master, NULL, NULL, NULL, NULL, NULL,
-	  dpn, n, dout, (unsigned)G.cus * 4u, G.stream))) {
+	  dpn, n, 1, dout, (unsigned)G.cus * 4u, G.stream))) {
 		r = rl_hipfail(r, "guide mixture");
 		goto out;
 	}
@@ -546,7 +562,7 @@ int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned in
 		}
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, dcnt, n, first,
-		  chunk, dsc, cdf, g->grad, dsc + 1, (unsigned)G.cus * 4u, G.stream))) {
+		  chunk, dsc, cdf, g->grad, dsc + 1, (unsigned)G.cus * 2u, G.stream))) {
 			r = rl_hipfail(r, "guide train");
 			goto out;
 		}

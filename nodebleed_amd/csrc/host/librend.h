@@ -50,6 +50,8 @@ struct rl_guide {
 	float     mix_prob, lr_grid, lr_mlp;
 	unsigned  stride, steps, step, last_records;
 	unsigned  chunk;                    /* records per training step */
+	unsigned  depth;                    /* vertices at depth < depth are guided */
+	int       share;                    /* one depth-0 proposal per pixel */
 	int       importance;               /* draw records in proportion to target */
 	int       enabled;
 };
@@ -100,11 +102,11 @@ struct PtArgs;
 void             rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap);
 int              rl_guide_buffers(struct rl_scene *s, size_t cap);
 int              rl_guide_eval(struct rl_scene *s, const struct PtArgs *a,
-                               const void *rays, const void *hits,
-                               const unsigned *counts, hipStream_t st);
+                                const void *rays, const void *hits,
+                                const unsigned *counts, unsigned share, hipStream_t st);
 int              rl_guide_train(struct rl_scene *s, const struct PtArgs *a,
                                 const void *paths, unsigned *counts,
-                                hipStream_t st);
+                                unsigned steps, hipStream_t st);
 int              rl_sync_scene(struct rl_scene *s);
 int              rl_render_pt(struct rl_scene *s, unsigned rw, unsigned rh,
                               unsigned *rays);

@@ -283,7 +283,11 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 		return r;
 	float *mix = guided && a->guide_mix > 0.0f ? s->guide->mix : NULL;
 	void *rec = guided && a->train_stride ? s->guide->rec : NULL;
-	for (unsigned long long first = 0; first < total; first += cap) {
+	/* the frame's training steps are spread over its batches */
+	const unsigned long long nb = (total + cap - 1) / cap;
+	const unsigned steps = rec ? s->guide->steps : 0;
+	unsigned long long b = 0;
+	for (unsigned long long first = 0; first < total; first += cap, b++) {
 		unsigned n = (unsigned)(total - first < cap ? total - first : cap);
 		if ((r = rk_pt_gen(a, t->views, t->paths, t->rays[0], t->q[0], t->counts,
 		  (unsigned)first, n, st)))
@@ -296,7 +300,8 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, RK_TRACE_NUDGE, st);
 			if (r)
 				return rl_hipfail(r, "extend trace");
-			if (mix && (r = rl_guide_eval(s, a, t->rays[cur], t->hits, t->counts, st)))
+			if (mix && d < a->guide_depth && (r = rl_guide_eval(s, a, t->rays[cur], t->hits,
+			  t->counts, d == 0 ? a->guide_share : 1u, st)))
 				return r;
 			if ((r = rk_pt_shade(a, t->paths, t->rays[cur], t->hits, t->q[cur],
 			  t->rays[cur ^ 1], t->q[cur ^ 1], t->srays, t->spath, t->scontrib,
@@ -319,7 +324,8 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 		}
 		if ((r = rk_pt_accum(t->paths, n, acc, st)))
 			return rl_hipfail(r, "accumulate");
-		if (rec && (r = rl_guide_train(s, a, t->paths, t->counts, st)))
+		const unsigned sb = (unsigned)((steps * (b + 1)) / nb - (steps * b) / nb);
+		if (rec && sb && (r = rl_guide_train(s, a, t->paths, t->counts, sb, st)))
 			return r;
 	}
 	return 0;
@@ -398,6 +404,11 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 		return rl_hipfail(e, "view upload");
 	struct PtArgs a;
 	pt_args(s, &a, nv, 0x68bc21ebu);
+	/* one view whose pixels' samples are consecutive paths of one batch:
+	 * its depth-0 proposal is evaluated once per pixel */
+	if (a.guide_mix > 0.0f && s->guide && s->guide->share && nv == 1 &&
+	    views[0].spp > 1 && t->batch % views[0].spp == 0)
+		a.guide_share = views[0].spp;
 	/* allocations (spill area) happen before the key and any capture */
 	unsigned g0;
 	if ((r = rl_trace_grid(1, 0, &g0)) || (r = rl_trace_grid(0, 0, &g0)))

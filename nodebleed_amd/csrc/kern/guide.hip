@@ -333,164 +333,6 @@ __global__ __launch_bounds__(256) void k_guide_tile_scan(const float *__restrict
     }
 }
 
-// Forward + backward over `chunk` records drawn from the batch: gradients of
-// the mean of -(target / mean target) log p(dir) into grad (fp32, atomics).
-__global__ __launch_bounds__(64) void k_guide_train(PtArgs a, Net N,
-                                                    const GdRecord *__restrict__ rec,
-                                                    const uint32_t *__restrict__ counts,
-                                                    uint32_t cap, uint32_t first,
-                                                    uint32_t chunk,
-                                                    const float *__restrict__ wsum,
-                                                    const uint64_t *__restrict__ cdf,
-                                                    float *__restrict__ grad,
-                                                    float *__restrict__ loss) {
-  __shared__ WaveLds S;
-  __shared__ __attribute__((aligned(16))) __bf16 dout[kWave * GD_OUT];
-  __shared__ __attribute__((aligned(16))) __bf16 dh[kWave * GD_HID];
-  const uint32_t lane = lane_id();
-  // records of this step: [first, first + n) within the frame's valid records
-  const uint32_t total = counts ? min(counts[PT_CNT_TRAIN], cap) : cap;
-  // `first` seeds the step's sample: records (i * P + first) mod total for
-  // i < n, with P a prime, are distinct and spread over the whole batch
-  // (records are stored in queue order, i.e. roughly pixel order)
-  const uint32_t n = min(chunk, total);
-  const float wmean = wsum[0] / (float)(total ? total : 1u);
-  const float inv = wmean > 0.0f ? 1.0f / (wmean * (float)(n ? n : 1u)) : 0.0f;
-  // With a cdf: systematic sampling of n records with probability
-  // target / sum (offset (first >> 8) / 2^24 within each of the n strata),
-  // each then weighted 1/n -- the same expected gradient with far less
-  // variance when a few records carry most of the radiance.
-  const uint64_t ctot = (cdf && total) ? cdf[total - 1] : 0ull;
-  const double cstep = (double)ctot / (double)(n ? n : 1u);
-  const double xi = (double)(first >> 8) * (1.0 / 16777216.0);
-  float lacc = 0.0f;
-  for (uint32_t base = blockIdx.x * kWave; base < n; base += gridDim.x * kWave) {
-    const uint32_t i = base + lane;
-    bool valid = i < n;
-    GdRecord r;
-    float wgt = 0.0f;
-    if (valid && cdf) {
-      const uint64_t u = (uint64_t)(((double)i + xi) * cstep);
-      uint32_t lo = 0, hi = total;
-      while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cdf[mid] <= u) lo = mid + 1; else hi = mid;
-      }
-      r = rec[lo < total ? lo : total - 1];
-      valid = ctot > 0 && r.target > 0.0f;
-      wgt = valid ? 1.0f / (float)n : 0.0f;
-    } else if (valid) {
-      r = rec[((uint64_t)i * 2654435761ull + first) % total];
-      wgt = r.target * inv;
-    }
-    if (!valid) {
-      for (int k = 0; k < 3; k++) { r.pos[k] = 0.0f; r.nrm[k] = 0.0f; r.dir[k] = 0.0f; }
-      r.nrm[2] = 1.0f; r.dir[2] = 1.0f; r.target = 0.0f;
-    }
-    encode(a, N.grid, r.pos, r.nrm, &S.act0[lane * GD_IN]);
-    __syncthreads();
-    forward(N, S.act0, S.act1, S.act2, S.act3, S.out);
-    // d loss / d raw outputs for this lane's sample
-    float g[GD_OUT];
-    lacc += gd_loss_grad(&S.out[lane * GD_OUT], r.dir, wgt, g);
-    for (int k = 0; k < GD_OUT; k++) dout[lane * GD_OUT + k] = (__bf16)g[k];
-    __syncthreads();
-    // bias gradients: column sums of dout
-    for (int k = 0; k < GD_OUT; k++) {
-      float v = g[k];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-      if (lane == 0 && v != 0.0f) atomicAdd(&grad[GD_OFS_B4 + k], v);
-    }
-    // Backward through the four layers.  For layer j with input X (64 x K),
-    // weight W (N x K, out-major) and output gradient D (64 x N):
-    //   dW[n][k] = sum_m D[m][n] X[m][k];  dX[m][k] = sum_n D[m][n] W[n][k].
-    const __bf16 *Xs[4];
-    Xs[0] = S.act0;
-    Xs[1] = S.act1;
-    Xs[2] = S.act2;
-    Xs[3] = S.act3;
-    const int Ks[4] = {GD_IN, GD_HID, GD_HID, GD_HID};
-    const int Ns[4] = {GD_HID, GD_HID, GD_HID, GD_OUT};
-    const uint32_t wofs[4] = {GD_OFS_W1, GD_OFS_W2, GD_OFS_W3, GD_OFS_W4};
-    const uint32_t bofs[4] = {GD_OFS_B1, GD_OFS_B2, GD_OFS_B3, GD_OFS_B4};
-    const __bf16 *D = dout;
-    for (int j = 3; j >= 0; j--) {
-      const int K = Ks[j], Nn = Ns[j];
-      // dW (N x K) = D^T X : A(n, m) = D[m*Nn + n], B(m, k) = X[m*K + k]
-      for (int rt = 0; rt < Nn; rt += 16)
-        for (int ct = 0; ct < K; ct += 16) {
-          const f32x4 d = tile_mm(D, 1, Nn, Xs[j], K, 1, rt, ct, kWave);
-          const int k = ct + (int)(lane & 15u);
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int nrow = rt + 4 * (int)(lane >> 4) + q;
-            if (d[q] != 0.0f) atomicAdd(&grad[wofs[j] + nrow * K + k], d[q]);
-          }
-        }
-      if (j == 0) {
-        // dX for the encoding inputs (first 24 columns)
-        for (int rt = 0; rt < kWave; rt += 16)
-          for (int ct = 0; ct < GD_IN; ct += 16) {
-            const f32x4 d = tile_mm(D, Nn, 1, wptr(N, 0), K, 1, rt, ct, Nn);
-            const int k = ct + (int)(lane & 15u);
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int m = rt + 4 * (int)(lane >> 4) + q;
-              S.out[m * GD_OUT + k] = d[q];  // reuse as fp32 scratch
-            }
-          }
-        __syncthreads();
-        // scatter into the hash grid
-        if (valid) {
-          float q3[3];
-          normalise_pos(a, r.pos, q3);
-          for (int l = 0; l < GD_LEVELS; l++) {
-            const float d0 = S.out[lane * GD_OUT + 2 * l], d1 = S.out[lane * GD_OUT + 2 * l + 1];
-            if (d0 == 0.0f && d1 == 0.0f) continue;
-            for (int c = 0; c < 8; c++) {
-              uint32_t idx;
-              float w;
-              grid_corner(l, q3, c, idx, w);
-              const size_t o = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
-              atomicAdd(&grad[o], w * d0);
-              atomicAdd(&grad[o + 1], w * d1);
-            }
-          }
-        }
-        __syncthreads();
-        break;
-      }
-      // dX (64 x K) = D W, then the ReLU mask of X (X = relu output)
-      __bf16 *nd = dh;
-      for (int rt = 0; rt < kWave; rt += 16)
-        for (int ct = 0; ct < K; ct += 16) {
-          const f32x4 d = tile_mm(D, Nn, 1, wptr(N, j), K, 1, rt, ct, Nn);
-          const int k = ct + (int)(lane & 15u);
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            const int m = rt + 4 * (int)(lane >> 4) + q;
-            const float x = (float)Xs[j][m * K + k];
-            S.out[m * GD_OUT + k] = x > 0.0f ? d[q] : 0.0f;
-          }
-        }
-      __syncthreads();
-      // bias gradient of the previous layer and bf16 copy for the next step
-      for (int k = 0; k < K; k++) {
-        float v = S.out[lane * GD_OUT + k];
-        nd[lane * K + k] = (__bf16)v;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-        if (lane == 0 && v != 0.0f) atomicAdd(&grad[bofs[j - 1] + k], v);
-      }
-      __syncthreads();
-      // move dh into dout so the next iteration reads it as D
-      for (int k = 0; k < K; k++) dout[lane * GD_HID + k] = nd[lane * K + k];
-      __syncthreads();
-      D = dout;
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) lacc += __shfl_down(lacc, o);
-  if (lane == 0 && loss) atomicAdd(loss, lacc);
-}
 
 // ---- register-resident inference ------------------------------------------
 //
@@ -564,6 +406,435 @@ __device__ __forceinline__ void layer64(const __bf16 *__restrict__ W, const bf16
   }
 }
 
+// ---- training kernel ------------------------------------------------------
+//
+// Same register layout as k_guide_mix: a wave owns 16 samples, lane l holds
+// sample s = l % 16 and k-group g = l / 16; activations stay in MFMA D/B
+// fragments.  Backward products W^T dY use a third bf16 mirror of the
+// weights, transposed and with the output units in the permuted K order
+// (w16 + 2 GD_NW16: W1T [32][64], W2T [64][64], W3T [64][64], W4T [64][64]).
+// Weight gradients sum over the wave's 16 samples with
+// v_mfma_f32_16x16x16_bf16 after a 16 x 16 transpose through LDS, and are
+// accumulated per block in LDS (fp32) before one global flush.
+
+__device__ __forceinline__ int permT_dst(uint32_t i) {
+  if (i >= GD_OFS_W1 && i < GD_OFS_B1) {
+    const uint32_t k = i - GD_OFS_W1;
+    return (int)((k % GD_IN) * GD_HID + perm_col(k / GD_IN));
+  }
+  if (i >= GD_OFS_W2 && i < GD_OFS_B2) {
+    const uint32_t k = i - GD_OFS_W2;
+    return (int)(GD_IN * GD_HID + (k % GD_HID) * GD_HID + perm_col(k / GD_HID));
+  }
+  if (i >= GD_OFS_W3 && i < GD_OFS_B3) {
+    const uint32_t k = i - GD_OFS_W3;
+    return (int)(GD_IN * GD_HID + GD_HID * GD_HID + (k % GD_HID) * GD_HID + perm_col(k / GD_HID));
+  }
+  if (i >= GD_OFS_W4 && i < GD_OFS_B4) {
+    const uint32_t k = i - GD_OFS_W4, o = k / GD_HID;
+    if (o >= GD_MIX) return -1;
+    return (int)(GD_IN * GD_HID + 2 * GD_HID * GD_HID + (k % GD_HID) * GD_HID +
+                 perm_col(perm_out_row(o)));
+  }
+  return -1;
+}
+
+typedef __attribute__((__vector_size__(4 * sizeof(short)))) short s16x4;
+
+constexpr int kTrainWaves = 4;
+constexpr uint32_t kDwW1 = 0, kDwW2 = GD_HID * GD_IN, kDwW3 = kDwW2 + GD_HID * GD_HID,
+                   kDwW4 = kDwW3 + GD_HID * GD_HID, kDwN = kDwW4 + GD_MIX * GD_HID;
+constexpr uint32_t kDbB1 = 0, kDbB2 = GD_HID, kDbB3 = 2 * GD_HID, kDbB4 = 3 * GD_HID,
+                   kDbN = 3 * GD_HID + GD_MIX;
+
+struct TrainLds {
+  float dw[kDwN];                                        // weight gradients (fp32)
+  float db[kDbN];                                        // bias gradients
+  __attribute__((aligned(16))) __bf16 ty[kTrainWaves][GD_HID * 16];  // dY rows x 16 samples
+  __attribute__((aligned(16))) __bf16 tx[kTrainWaves][GD_HID * 16];  // X rows x 16 samples
+};
+
+// Orders the wave's LDS accesses (lanes of one wave exchange data).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Hidden unit of K position 32 kb + 8 g + j of a packed B fragment.
+__device__ __forceinline__ uint32_t kpos_unit(uint32_t kb, uint32_t g, uint32_t j) {
+  return 16u * (2u * kb + (j >> 2)) + 4u * g + (j & 3u);
+}
+
+// D fragments of tiles 2 kb, 2 kb + 1 packed as a B operand (no bias).
+__device__ __forceinline__ bf16x8 pack_b(const f32x4 &d0, const f32x4 &d1) {
+  bf16x8 x;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    x[r] = (__bf16)d0[r];
+    x[4 + r] = (__bf16)d1[r];
+  }
+  return x;
+}
+
+// X rows (units of a packed B operand pair) into the wave's transpose tile.
+__device__ __forceinline__ void put_x(__bf16 *tx, const bf16x8 xb[2], uint32_t s, uint32_t g) {
+#pragma unroll
+  for (uint32_t kb = 0; kb < 2; kb++)
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) tx[kpos_unit(kb, g, j) * 16u + s] = xb[kb][j];
+}
+
+// dY rows (D fragments of 4 tiles) into the wave's transpose tile.
+__device__ __forceinline__ void put_y(__bf16 *ty, const f32x4 d[4], uint32_t s, uint32_t g) {
+#pragma unroll
+  for (uint32_t t = 0; t < 4; t++)
+#pragma unroll
+    for (uint32_t r = 0; r < 4; r++) ty[(16u * t + 4u * g + r) * 16u + s] = (__bf16)d[t][r];
+}
+
+// dW[n][k] += sum over the 16 samples of Y[n][s] X[k][s] (LDS accumulator,
+// row stride ldk; rows n >= nvalid are skipped).
+__device__ __forceinline__ void dw_accum(const __bf16 *ty, const __bf16 *tx, int nrows,
+                                         int nvalid, int krows, float *dw, int ldk) {
+  const uint32_t l = lane_id(), c = l & 15u, g = l >> 4;
+  for (int nt = 0; nt < nrows; nt += 16) {
+    const s16x4 av = *(const s16x4 *)(ty + (nt + (int)c) * 16 + 4 * (int)g);
+    for (int kt = 0; kt < krows; kt += 16) {
+      const s16x4 bv = *(const s16x4 *)(tx + (kt + (int)c) * 16 + 4 * (int)g);
+      const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+      const f32x4 d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, z, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int n = nt + 4 * (int)g + r;
+        if (n < nvalid && d[r] != 0.0f) atomicAdd(&dw[n * ldk + kt + (int)c], d[r]);
+      }
+    }
+  }
+}
+
+// Bias gradient: sum of v over the 16 samples of the lane's k-group.
+__device__ __forceinline__ float sum16(float v) {
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 8);
+  return v;
+}
+
+// Mask by the ReLU of the forward activations xb (packed B layout) and
+// return as D fragments + packed B operands for the next product.
+__device__ __forceinline__ void relu_back(f32x4 d[4], const bf16x8 xb[2]) {
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if (!((float)xb[t >> 1][4 * (t & 1) + r] > 0.0f)) d[t][r] = 0.0f;
+}
+
+// D (4 tiles of 16 units) = W^T dY for a 64-unit K with the transposed mirror.
+__device__ __forceinline__ void back64(const __bf16 *__restrict__ WT, const bf16x8 yb[2],
+                                       uint32_t s, uint32_t g, int ntiles, f32x4 *d) {
+  for (int t = 0; t < ntiles; t++) {
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int kb = 0; kb < 2; kb++) {
+      const bf16x8 av = *(const bf16x8 *)(WT + (16u * t + s) * GD_HID + 32u * kb + 8u * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, yb[kb], acc, 0, 0, 0);
+    }
+    d[t] = acc;
+  }
+}
+
+// Forward + backward over n = min(chunk, records) samples: gradients of the
+// mean of -(target / mean target) log p(dir) (uniformly drawn records) or of
+// the mean of -log p(dir) (records drawn in proportion to target) into grad.
+__global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__restrict__ grid,
+                                                     const __bf16 *__restrict__ wp,
+                                                     const __bf16 *__restrict__ wt,
+                                                     const float *__restrict__ master,
+                                                     const GdRecord *__restrict__ rec,
+                                                     const uint32_t *__restrict__ counts,
+                                                     uint32_t cap, uint32_t first,
+                                                     uint32_t chunk,
+                                                     const float *__restrict__ wsum,
+                                                     const uint64_t *__restrict__ cdf,
+                                                     float *__restrict__ grad,
+                                                     float *__restrict__ loss) {
+  __shared__ TrainLds S;
+  for (uint32_t k = threadIdx.x; k < kDwN; k += blockDim.x) S.dw[k] = 0.0f;
+  for (uint32_t k = threadIdx.x; k < kDbN; k += blockDim.x) S.db[k] = 0.0f;
+  __syncthreads();
+  const uint32_t lane = lane_id(), s = lane & 15u, g = lane >> 4;
+  const uint32_t wv = threadIdx.x >> 6;
+  __bf16 *ty = S.ty[wv], *tx = S.tx[wv];
+  const uint32_t total = counts ? min(counts[PT_CNT_TRAIN], cap) : cap;
+  const uint32_t n = min(chunk, total);
+  const float wmean = wsum[0] / (float)(total ? total : 1u);
+  const float inv = wmean > 0.0f ? 1.0f / (wmean * (float)(n ? n : 1u)) : 0.0f;
+  const uint64_t ctot = (cdf && total) ? cdf[total - 1] : 0ull;
+  const double cstep = (double)ctot / (double)(n ? n : 1u);
+  const double xi = (double)(first >> 8) * (1.0 / 16777216.0);
+  float lacc = 0.0f;
+  const uint32_t ntiles = (n + 15u) / 16u;
+  const uint32_t wave = blockIdx.x * kTrainWaves + wv, nw = gridDim.x * kTrainWaves;
+  for (uint32_t tile = wave; tile < ntiles; tile += nw) {
+    // weight fragments are re-read per tile (from the caches); the opaque
+    // zero offset stops the compiler from hoisting all of them into
+    // registers
+    uint32_t wofs = 0;
+    asm volatile("" : "+s"(wofs));
+    const __bf16 *W1 = wp + wofs, *W2 = W1 + GD_HID * GD_IN, *W3 = W2 + GD_HID * GD_HID,
+                 *W4 = W3 + GD_HID * GD_HID;
+    const __bf16 *W1T = wt + wofs, *W2T = W1T + GD_IN * GD_HID, *W3T = W2T + GD_HID * GD_HID,
+                 *W4T = W3T + GD_HID * GD_HID;
+    const float *mp = master + wofs;
+    const uint32_t i = tile * 16u + s;
+    bool valid = i < n;
+    GdRecord r;
+    float wgt = 0.0f;
+    if (valid && cdf) {
+      // systematic draw in proportion to target (offset xi in each stratum)
+      const uint64_t u = (uint64_t)(((double)i + xi) * cstep);
+      uint32_t lo = 0, hi = total;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid] <= u) lo = mid + 1; else hi = mid;
+      }
+      r = rec[lo < total ? lo : total - 1];
+      valid = ctot > 0 && r.target > 0.0f;
+      wgt = valid ? 1.0f / (float)n : 0.0f;
+    } else if (valid) {
+      r = rec[((uint64_t)i * 2654435761ull + first) % total];
+      wgt = r.target * inv;
+    }
+    if (!valid || !(wgt > 0.0f) || !(wgt < 1e30f)) {
+      for (int k = 0; k < 3; k++) { r.pos[k] = 0.0f; r.nrm[k] = 0.0f; r.dir[k] = 0.0f; }
+      r.nrm[2] = 1.0f; r.dir[2] = 1.0f;
+      wgt = 0.0f;
+      valid = false;
+    }
+    float q[3];
+    normalise_pos(a, r.pos, q);
+    // ---- forward (as k_guide_mix), keeping every layer's input
+    bf16x8 x0;
+    if (g < 3u) {
+#pragma unroll
+      for (int lv = 0; lv < 4; lv++) {
+        const int l = 4 * (int)g + lv;
+        float f0 = 0.0f, f1 = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          uint32_t idx;
+          float w;
+          grid_corner(l, q, c, idx, w);
+          const __half2 gv = *(const __half2 *)&grid[((size_t)l * GD_TABLE + idx) * GD_FEAT];
+          f0 += w * __low2float(gv);
+          f1 += w * __high2float(gv);
+        }
+        x0[2 * lv] = (__bf16)f0;
+        x0[2 * lv + 1] = (__bf16)f1;
+      }
+    } else {
+      x0[0] = (__bf16)r.nrm[0];
+      x0[1] = (__bf16)r.nrm[1];
+      x0[2] = (__bf16)r.nrm[2];
+      x0[3] = (__bf16)1.0f;
+      x0[4] = x0[5] = x0[6] = x0[7] = (__bf16)0.0f;
+    }
+    f32x4 h[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const bf16x8 av = *(const bf16x8 *)(W1 + (16u * t + s) * GD_IN + 8u * g);
+      const f32x4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+      h[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, x0, z, 0, 0, 0);
+    }
+    bf16x8 x1[2], x2[2], x3[2];
+    x1[0] = relu_pack(h[0], h[1], mp + GD_OFS_B1, 0u, g);
+    x1[1] = relu_pack(h[2], h[3], mp + GD_OFS_B1, 2u, g);
+    layer64(W2, x1, s, g, h);
+    x2[0] = relu_pack(h[0], h[1], mp + GD_OFS_B2, 0u, g);
+    x2[1] = relu_pack(h[2], h[3], mp + GD_OFS_B2, 2u, g);
+    layer64(W3, x2, s, g, h);
+    x3[0] = relu_pack(h[0], h[1], mp + GD_OFS_B3, 0u, g);
+    x3[1] = relu_pack(h[2], h[3], mp + GD_OFS_B3, 2u, g);
+    float o[12];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int kb = 0; kb < 2; kb++) {
+        const bf16x8 av = *(const bf16x8 *)(W4 + (16u * t + s) * GD_HID + 32u * kb + 8u * g);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, x3[kb], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) o[4 * t + rr] = acc[rr];
+    }
+#pragma unroll
+    for (int sl = 0; sl < 10; sl++) o[sl] += mp[GD_OFS_B4 + 10u * g + (uint32_t)sl];
+    // ---- loss gradient for this lane's lobes 2g, 2g + 1 (slots 0..4, 5..9)
+    float mx = fmaxf(o[0], o[5]);
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    const float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
+    float esum = e0 + e1;
+    esum += __shfl_xor(esum, 16);
+    esum += __shfl_xor(esum, 32);
+    float pi[2], mu[2][3], kap[2], cth[2], vv[2], il[2];
+    bool dirok[2];
+#pragma unroll
+    for (int lb = 0; lb < 2; lb++) {
+      const float *ol = &o[5 * lb];
+      const float x = ol[1], y = ol[2], z = ol[3];
+      const float l2 = x * x + y * y + z * z;
+      dirok[lb] = l2 >= 1e-12f;
+      il[lb] = dirok[lb] ? 1.0f / sqrtf(l2) : 0.0f;
+      mu[lb][0] = dirok[lb] ? x * il[lb] : 0.0f;
+      mu[lb][1] = dirok[lb] ? y * il[lb] : 0.0f;
+      mu[lb][2] = dirok[lb] ? z * il[lb] : 1.0f;
+      kap[lb] = expf(fminf(fmaxf(ol[4], GD_LOGK_MIN), GD_LOGK_MAX));
+      pi[lb] = (lb ? e1 : e0) / esum;
+      cth[lb] = mu[lb][0] * r.dir[0] + mu[lb][1] * r.dir[1] + mu[lb][2] * r.dir[2];
+      vv[lb] = gd_vmf(kap[lb], cth[lb]);
+    }
+    float pd = pi[0] * vv[0] + pi[1] * vv[1];
+    pd += __shfl_xor(pd, 16);
+    pd += __shfl_xor(pd, 32);
+    pd = fmaxf(pd, 1e-20f);
+    float go[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) go[k] = 0.0f;
+    if (wgt > 0.0f) {
+#pragma unroll
+      for (int lb = 0; lb < 2; lb++) {
+        const float rsp = pi[lb] * vv[lb] / pd;  // responsibility
+        go[5 * lb] = -wgt * (rsp - pi[lb]);
+        if (dirok[lb]) {
+          const float sc = -wgt * rsp * kap[lb] * il[lb];
+#pragma unroll
+          for (int k = 0; k < 3; k++) go[5 * lb + 1 + k] = sc * (r.dir[k] - cth[lb] * mu[lb][k]);
+        }
+        const float lk = o[5 * lb + 4];
+        if (lk > GD_LOGK_MIN && lk < GD_LOGK_MAX) {
+          const float dl = 1.0f / kap[lb] - 2.0f / expm1f(2.0f * kap[lb]) + (cth[lb] - 1.0f);
+          go[5 * lb + 4] = -wgt * rsp * kap[lb] * dl;
+        }
+      }
+      if (g == 0) lacc -= wgt * logf(pd);
+    }
+    // ---- output layer: dW4, db4, dX3
+#pragma unroll
+    for (int sl = 0; sl < 10; sl++) ty[(10u * g + (uint32_t)sl) * 16u + s] = (__bf16)go[sl];
+    put_x(tx, x3, s, g);
+    wave_lds_sync();
+    dw_accum(ty, tx, 48, GD_MIX, GD_HID, S.dw + kDwW4, GD_HID);
+    wave_lds_sync();
+#pragma unroll
+    for (int sl = 0; sl < 10; sl++) {
+      const float v = sum16(go[sl]);
+      if (s == 0 && v != 0.0f) atomicAdd(&S.db[kDbB4 + 10u * g + (uint32_t)sl], v);
+    }
+    bf16x8 yb[2];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      yb[0][j] = (__bf16)go[j];
+      yb[1][j] = (__bf16)(j < 2 ? go[8 + j] : 0.0f);
+    }
+    f32x4 d[4];
+    back64(W4T, yb, s, g, 4, d);
+    relu_back(d, x3);
+    // ---- hidden layers 3 and 2: dW, db, dX
+#pragma unroll
+    for (int layer = 3; layer >= 2; layer--) {
+      const bf16x8 *xin = layer == 3 ? x2 : x1;
+      const bf16x8 *xmask = layer == 3 ? x2 : x1;
+      put_y(ty, d, s, g);
+      put_x(tx, xin, s, g);
+      wave_lds_sync();
+      dw_accum(ty, tx, GD_HID, GD_HID, GD_HID, S.dw + (layer == 3 ? kDwW3 : kDwW2), GD_HID);
+      wave_lds_sync();
+      const uint32_t bo = layer == 3 ? kDbB3 : kDbB2;
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          const float v = sum16(d[t][rr]);
+          if (s == 0 && v != 0.0f) atomicAdd(&S.db[bo + 16u * t + 4u * g + rr], v);
+        }
+      yb[0] = pack_b(d[0], d[1]);
+      yb[1] = pack_b(d[2], d[3]);
+      back64(layer == 3 ? W3T : W2T, yb, s, g, 4, d);
+      relu_back(d, xmask);
+    }
+    // ---- layer 1: dW1, db1, dX0 (inputs; the first 24 are grid features)
+    put_y(ty, d, s, g);
+#pragma unroll
+    for (uint32_t j = 0; j < 8; j++) tx[(8u * g + j) * 16u + s] = x0[j];
+    wave_lds_sync();
+    dw_accum(ty, tx, GD_HID, GD_HID, GD_IN, S.dw + kDwW1, GD_IN);
+    wave_lds_sync();
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const float v = sum16(d[t][rr]);
+        if (s == 0 && v != 0.0f) atomicAdd(&S.db[kDbB1 + 16u * t + 4u * g + rr], v);
+      }
+    yb[0] = pack_b(d[0], d[1]);
+    yb[1] = pack_b(d[2], d[3]);
+    f32x4 dx[2];
+    back64(W1T, yb, s, g, 2, dx);
+    // input rows 16 t + 4 g + rr: pairs (rr 0,1) and (rr 2,3) are the two
+    // features of level (16 t + 4 g + rr) / 2
+    if (valid) {
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int pr = 0; pr < 2; pr++) {
+          const uint32_t jin = 16u * t + 4u * g + 2u * pr;
+          if (jin >= 2u * GD_LEVELS) continue;
+          const float d0 = dx[t][2 * pr], d1 = dx[t][2 * pr + 1];
+          if (d0 == 0.0f && d1 == 0.0f) continue;
+          const int l = (int)(jin >> 1);
+          for (int c = 0; c < 8; c++) {
+            uint32_t idx;
+            float w;
+            grid_corner(l, q, c, idx, w);
+            const size_t go2 = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
+            atomicAdd(&grad[go2], w * d0);
+            atomicAdd(&grad[go2 + 1], w * d1);
+          }
+        }
+    }
+  }
+  if (loss) {
+    for (int o2 = 32; o2 > 0; o2 >>= 1) lacc += __shfl_down(lacc, o2);
+    if (lane == 0 && lacc != 0.0f) atomicAdd(loss, lacc);
+  }
+  __syncthreads();
+  // flush the block's weight and bias gradients
+  for (uint32_t k = threadIdx.x; k < kDwN; k += blockDim.x) {
+    const float v = S.dw[k];
+    if (v == 0.0f) continue;
+    uint32_t dst;
+    if (k < kDwW2) dst = GD_OFS_W1 + k;
+    else if (k < kDwW3) dst = GD_OFS_W2 + (k - kDwW2);
+    else if (k < kDwW4) dst = GD_OFS_W3 + (k - kDwW3);
+    else dst = GD_OFS_W4 + (k - kDwW4);
+    atomicAdd(&grad[dst], v);
+  }
+  for (uint32_t k = threadIdx.x; k < kDbN; k += blockDim.x) {
+    const float v = S.db[k];
+    if (v == 0.0f) continue;
+    uint32_t dst;
+    if (k < kDbB2) dst = GD_OFS_B1 + k;
+    else if (k < kDbB3) dst = GD_OFS_B2 + (k - kDbB2);
+    else if (k < kDbB4) dst = GD_OFS_B3 + (k - kDbB3);
+    else dst = GD_OFS_B4 + (k - kDbB4);
+    atomicAdd(&grad[dst], v);
+  }
+}
+
 // Mixture records (GD_MIX floats) of queued hit points (pn == nullptr) or of
 // n position/normal pairs pn[6 i ..].
 __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__restrict__ grid,
@@ -575,8 +846,10 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
                                                    const dinst *__restrict__ insts,
                                                    const RTri *__restrict__ tris,
                                                    const float *__restrict__ pn, uint32_t npn,
-                                                   float *__restrict__ mix) {
-  const uint32_t n = pn ? npn : counts[PT_CNT_CUR];
+                                                   uint32_t share, float *__restrict__ mix) {
+  // queue entries 0, share, 2 share, ... -> records 0, 1, 2, ...
+  const uint32_t nq = pn ? npn : counts[PT_CNT_CUR];
+  const uint32_t n = (nq + share - 1u) / share;
   const uint32_t lane = lane_id();
   const uint32_t s = lane & 15u, g = lane >> 4;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -597,7 +870,8 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
         for (int k = 0; k < 3; k++) { p[k] = pn[6 * i + k]; nn[k] = pn[6 * i + 3 + k]; }
         ok = true;
       } else {
-        ok = hit_point(rays[i], hits[i], insts, tris, p, nn);
+        const uint32_t qi = i * share;
+        ok = hit_point(rays[qi], hits[qi], insts, tris, p, nn);
       }
     }
     // layer-1 B operand: inputs 8g .. 8g+7 of sample s
@@ -683,7 +957,8 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
       rec[5 * lb + 3] = z;
       rec[5 * lb + 4] = expf(lk);
     }
-    if (i < n && ok) {
+    if (!ok) rec[0] = rec[5] = -1.0f;  // empty proposal
+    if (i < n) {
       float2 *dst = (float2 *)&mix[(size_t)i * GD_MIX + 10u * g];
 #pragma unroll
       for (int k = 0; k < 5; k++) dst[k] = make_float2(rec[2 * k], rec[2 * k + 1]);
@@ -727,6 +1002,8 @@ __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ gra
     if (dst >= 0) w16[dst] = (__bf16)master[i];
     const int pd = perm_dst(i);
     if (pd >= 0) w16[GD_NW16 + pd] = (__bf16)master[i];
+    const int td = permT_dst(i);
+    if (td >= 0) w16[2 * GD_NW16 + td] = (__bf16)master[i];
   }
 }
 
@@ -793,12 +1070,12 @@ static Net mknet(const void *grid16, const void *w16, const float *master) {
 extern "C" int rk_guide_mix(const PtArgs *a, const void *grid16, const void *w16,
                             const float *master, const void *rays, const void *hits,
                             const unsigned *counts, const void *insts, const void *tris,
-                            const float *pn, unsigned npn, float *mix, unsigned grid,
-                            void *stream) {
+                            const float *pn, unsigned npn, unsigned share, float *mix,
+                            unsigned grid, void *stream) {
   hipLaunchKernelGGL(k_guide_mix, dim3(grid), dim3(256), 0, (hipStream_t)stream, *a,
                      (const __half *)grid16, (const __bf16 *)w16 + GD_NW16, master,
                      (const rray *)rays, (const rhit *)hits, counts, (const dinst *)insts,
-                     (const RTri *)tris, pn, npn, mix);
+                     (const RTri *)tris, pn, npn, share ? share : 1u, mix);
   return (int)hipGetLastError();
 }
 
@@ -830,15 +1107,20 @@ extern "C" int rk_guide_cdf(const float *dense, const float *wsum, unsigned long
   return (int)hipGetLastError();
 }
 
+// `grid` bounds the number of 256-thread blocks (each holds 69 KB of LDS).
 extern "C" int rk_guide_train(const PtArgs *a, const void *grid16, const void *w16,
                               const float *master, const void *rec, const unsigned *counts,
                               unsigned cap, unsigned first, unsigned chunk, const float *wsum,
                               const unsigned long long *cdf, float *grad, float *loss,
                               unsigned grid, void *stream) {
   if (!chunk) return 0;
-  hipLaunchKernelGGL(k_guide_train, dim3(grid), dim3(64), 0, (hipStream_t)stream, *a,
-                     mknet(grid16, w16, master), (const GdRecord *)rec, counts, cap, first,
-                     chunk, wsum, (const uint64_t *)cdf, grad, loss);
+  const unsigned need = (chunk + 16u * kTrainWaves - 1) / (16u * kTrainWaves);
+  const unsigned nb = grid < need ? (grid ? grid : 1u) : need;
+  const __bf16 *w = (const __bf16 *)w16;
+  hipLaunchKernelGGL(k_guide_train, dim3(nb), dim3(64 * kTrainWaves), 0, (hipStream_t)stream,
+                     *a, (const __half *)grid16, w + GD_NW16, w + 2 * GD_NW16, master,
+                     (const GdRecord *)rec, counts, cap, first, chunk, wsum,
+                     (const uint64_t *)cdf, grad, loss);
   return (int)hipGetLastError();
 }
 

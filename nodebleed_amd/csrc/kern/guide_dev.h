@@ -15,6 +15,9 @@ __device__ __forceinline__ bool guide_sample(const float *mix, float alpha, floa
                                              float b2x, float b2y, float b2z, float u0,
                                              float u1, float u2, float &dx, float &dy,
                                              float &dz, float &wgt, float &pdf) {
+  // a proposal whose first weight is negative is empty (its point missed):
+  // BSDF sampling alone
+  if (mix[0] < 0.0f) alpha = 0.0f;
   if (u0 < alpha) {
     gd_sample(mix, u0 / alpha, u1, u2, dx, dy, dz);
   } else {

@@ -327,8 +327,10 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
             const float b2x = bb, b2y = sgn + ny * ny * aa, b2z = -ny;
             float dx, dy, dz, wgt;
             bool ok = true;
-            if (guide && a.guide_mix > 0.0f) {
-              ok = guide_sample(guide + (size_t)i * GD_MIX, a.guide_mix, nx, ny, nz, b1x, b1y,
+            if (guide && a.guide_mix > 0.0f && depth < a.guide_depth) {
+              // at depth 0 consecutive paths of a pixel share one proposal
+              const size_t gi = depth == 0 ? i / a.guide_share : i;
+              ok = guide_sample(guide + gi * GD_MIX, a.guide_mix, nx, ny, nz, b1x, b1y,
                                 b1z, b2x, b2y, b2z, rng_u01(key, dbase + 3),
                                 rng_u01(key, dbase + 4), rng_u01(key, dbase + 5), dx, dy, dz,
                                 wgt, p.pdf_prev);
@@ -356,7 +358,8 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
                 want_next = true;
                 nro[0] = px + dx * kEps; nro[1] = py + dy * kEps; nro[2] = pz + dz * kEps;
                 nrd[0] = dx; nrd[1] = dy; nrd[2] = dz;
-                if (train && a.train_stride && (pi % a.train_stride) == 0) {
+                if (train && a.train_stride && (pi % a.train_stride) == 0 &&
+                    depth < a.guide_depth) {
                   want_rec = true;
                   rec.pos[0] = px; rec.pos[1] = py; rec.pos[2] = pz;
                   rec.nrm[0] = gx; rec.nrm[1] = gy; rec.nrm[2] = gz;

@@ -28,6 +28,8 @@ struct PtArgs {
 	float     ginv[3];        /* 1 / extent */
 	uint32_t  train_stride;   /* record vertices of every n-th path (0 = off) */
 	uint32_t  train_cap;      /* training records per batch */
+	uint32_t  guide_depth;    /* vertices at depth < guide_depth are guided */
+	uint32_t  guide_share;    /* depth 0: one proposal per this many paths */
 };
 
 /* Path state, 64 bytes. */
@@ -67,6 +69,8 @@ struct PtLight {
 #define GD_DEFAULT_CHUNK 32768u
 /* records drawn in proportion to their target (1) or uniformly (0) */
 #define GD_DEFAULT_IMPORTANCE 1
+/* guided vertices: depth < GD_DEFAULT_DEPTH */
+#define GD_DEFAULT_DEPTH 2
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 
@@ -86,6 +90,9 @@ struct PtLight {
 /* bf16 weight mirror: W1..W4 packed in natural order, then the same in the
  * permuted order of the register-resident inference kernel */
 #define GD_NW16    (GD_HID * GD_IN + 2 * GD_HID * GD_HID + GD_OUT * GD_HID)
+/* bf16 weight mirrors: plain, permuted (inference), transposed (backward) */
+#define GD_NW16T   (GD_IN * GD_HID + 3 * GD_HID * GD_HID)
+#define GD_W16_ELEMS (2 * GD_NW16 + GD_NW16T)
 
 /* Per shading point mixture: for each lobe weight, mean xyz, kappa. */
 #define GD_MIX     (5 * GD_LOBES)
