@@ -157,10 +157,14 @@ int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps)
  * drawn in proportion to their target (importance != 0) or uniformly. */
 int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
                                      float lr_grid, float lr_mlp, int importance);
-/* Which vertices are guided: those at depth < depth (default 2: the first
- * two bounces); share != 0 evaluates the depth-0 proposal once per pixel of
+/* Which vertices are guided: those at depth < depth (default 1: the
+ * primary hits); share != 0 evaluates the depth-0 proposal once per pixel of
  * a single view and uses it for all of the pixel's samples (default on). */
 int          rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share);
+/* Training target of a guided vertex: the radiance its path gathers after
+ * it (0, default) or only the next vertex's direct light (1: one bounce,
+ * lower variance). */
+int          rend_hip_guide_target(struct rdata *rd, int next_vertex_direct);
 /* out: last training loss, records of the last batch, Adam steps, enabled. */
 int          rend_hip_guide_info(struct rdata *rd, float out[4]);
 /* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),

@@ -24,7 +24,8 @@ int rk_guide_mix(const struct PtArgs *a, const void *grid16, const void *w16,
                  const float *pn, unsigned npn, unsigned share, float *mix,
                  unsigned grid, void *stream);
 int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
-                     float *wsum, float *dense, unsigned cap, void *stream);
+                     float *wsum, float *dense, unsigned cap, unsigned mode,
+                     void *stream);
 unsigned rk_guide_cdf_tiles(unsigned cap);
 int rk_guide_cdf(const float *dense, const float *wsum, unsigned long long *tsum,
                  unsigned long long *cdf, const unsigned *counts, unsigned cap,
@@ -196,6 +197,7 @@ int rend_hip_guide_enable(struct rdata *rd, float mix, unsigned int train_stride
 	g->importance = GD_DEFAULT_IMPORTANCE;
 	g->depth = GD_DEFAULT_DEPTH;
 	g->share = 1;
+	g->target = GD_DEFAULT_TARGET;
 	g->enabled = 1;
 	hipError_t e = hipStreamSynchronize(G.stream);
 	return e ? rl_hipfail(e, "guide enable") : 0;
@@ -237,6 +239,15 @@ int rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
 	g->lr_grid = lr_grid;
 	g->lr_mlp = lr_mlp;
 	g->importance = importance != 0;
+	return 0;
+}
+
+int rend_hip_guide_target(struct rdata *rd, int next_vertex_direct)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	s->guide->target = next_vertex_direct != 0;
 	return 0;
 }
 
@@ -344,7 +355,7 @@ int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths
 		return rl_hipfail(r, "guide scalars");
 	unsigned long long *cdf = g->importance ? g->cdf : NULL;
 	if ((r = rk_guide_targets(g->rec, counts, paths, g->scal, cdf ? g->dense : NULL, rc,
-	  st)))
+	  (unsigned)g->target, st)))
 		return rl_hipfail(r, "guide targets");
 	if (cdf && (r = rk_guide_cdf(g->dense, g->scal, g->tsum, cdf, counts, rc, st)))
 		return rl_hipfail(r, "guide record cdf");

@@ -52,6 +52,7 @@ struct rl_guide {
 	unsigned  chunk;                    /* records per training step */
 	unsigned  depth;                    /* vertices at depth < depth are guided */
 	int       share;                    /* one depth-0 proposal per pixel */
+	int       target;                   /* 1: train on next-vertex direct light */
 	int       importance;               /* draw records in proportion to target */
 	int       enabled;
 };

@@ -833,7 +833,7 @@ static int config_terrain(struct rs_scene *s, unsigned seed, float detail,
 		 * sit 0.03 below the ceiling, so each lit spot is small (radius
 		 * about 0.03) and bright: most of a terrain point's incident
 		 * radiance comes from a few small, well separated directions. */
-		int mwall = rs_material(s, 0.7f, 0.7f, 0.7f, 0);
+		int mwall = rs_material(s, 0.3f, 0.3f, 0.3f, 0);
 		int mceil = rs_material(s, 0.8f, 0.8f, 0.8f, 0);
 		int mlit = rs_material(s, 150.0f, 135.0f, 112.0f, RFLAG_EMISSIVE);
 		/* a room: ceiling, back and side walls, open towards the camera */

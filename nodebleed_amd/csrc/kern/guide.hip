@@ -211,14 +211,17 @@ struct WaveLds {
 // radiance; accumulates the sum of training weights into wsum.
 __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__restrict__ counts,
                                 const PtPath *__restrict__ paths, float *__restrict__ wsum,
-                                float *__restrict__ dense, uint32_t cap) {
+                                float *__restrict__ dense, uint32_t cap, uint32_t mode) {
   const uint32_t n = min(counts[PT_CNT_TRAIN], cap);
   float acc = 0.0f;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     GdRecord &r = rec[i];
     const PtPath &p = paths[r.path];
     const float Lf = 0.2126f * p.L[0] + 0.7152f * p.L[1] + 0.0722f * p.L[2];
-    float li = r.thr > 0.0f ? fmaxf(Lf - r.lbefore, 0.0f) / r.thr : 0.0f;
+    // mode 0: everything the path gathered after this vertex; mode 1: the
+    // next vertex's direct light only (lower variance, one bounce)
+    const float gathered = mode ? r.l1 : fmaxf(Lf - r.lbefore, 0.0f);
+    float li = r.thr > 0.0f ? gathered / r.thr : 0.0f;
     if (!(li < 1e6f)) li = 0.0f;
     const float c = fmaxf(r.nrm[0] * r.dir[0] + r.nrm[1] * r.dir[1] + r.nrm[2] * r.dir[2], 0.0f);
     r.target = r.pdf > 0.0f ? li * c / r.pdf : 0.0f;
@@ -785,27 +788,47 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
     f32x4 dx[2];
     back64(W1T, yb, s, g, 2, dx);
     // input rows 16 t + 4 g + rr: pairs (rr 0,1) and (rr 2,3) are the two
-    // features of level (16 t + 4 g + rr) / 2
-    if (valid) {
+    // features of level (16 t + 4 g + rr) / 2.  The 16 lanes of a k-group
+    // hold the same levels for 16 samples; where all of them (invalid
+    // samples count as agreeing) fall in one cell of a level -- the common
+    // case on coarse levels, where per-sample atomics would all contend for
+    // the same few addresses -- their corner gradients are summed across
+    // the group and added once.
 #pragma unroll
-      for (int t = 0; t < 2; t++)
+    for (int t = 0; t < 2; t++)
 #pragma unroll
-        for (int pr = 0; pr < 2; pr++) {
-          const uint32_t jin = 16u * t + 4u * g + 2u * pr;
-          if (jin >= 2u * GD_LEVELS) continue;
-          const float d0 = dx[t][2 * pr], d1 = dx[t][2 * pr + 1];
-          if (d0 == 0.0f && d1 == 0.0f) continue;
-          const int l = (int)(jin >> 1);
-          for (int c = 0; c < 8; c++) {
-            uint32_t idx;
-            float w;
-            grid_corner(l, q, c, idx, w);
+      for (int pr = 0; pr < 2; pr++) {
+        const uint32_t jin = 16u * t + 4u * g + 2u * pr;
+        if (jin >= 2u * GD_LEVELS) continue;  // k-group uniform
+        const float d0 = valid ? dx[t][2 * pr] : 0.0f, d1 = valid ? dx[t][2 * pr + 1] : 0.0f;
+        const int l = (int)(jin >> 1);
+        const float res = c_gd_res[l];
+        const uint32_t r1 = (uint32_t)res + 1u;
+        const uint32_t key = (uint32_t)floorf(q[0] * res) +
+                             r1 * ((uint32_t)floorf(q[1] * res) + r1 * (uint32_t)floorf(q[2] * res));
+        const uint32_t key0 = __shfl(key, (int)(lane & 48u));
+        const uint64_t agree = __ballot(!valid || key == key0);
+        const bool shared = ((agree >> (lane & 48u)) & 0xffffull) == 0xffffull;
+        for (int c = 0; c < 8; c++) {
+          uint32_t idx;
+          float w;
+          grid_corner(l, q, c, idx, w);
+          if (shared) {
+            const float v0 = sum16(w * d0), v1 = sum16(w * d1);
+            if (s == 0 && (v0 != 0.0f || v1 != 0.0f)) {
+              // lane 16 g holds a valid sample's corner unless the whole
+              // group is invalid (then v0 = v1 = 0)
+              const size_t go2 = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
+              atomicAdd(&grad[go2], v0);
+              atomicAdd(&grad[go2 + 1], v1);
+            }
+          } else if (d0 != 0.0f || d1 != 0.0f) {
             const size_t go2 = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
             atomicAdd(&grad[go2], w * d0);
             atomicAdd(&grad[go2 + 1], w * d1);
           }
         }
-    }
+      }
   }
   if (loss) {
     for (int o2 = 32; o2 > 0; o2 >>= 1) lacc += __shfl_down(lacc, o2);
@@ -1080,9 +1103,10 @@ extern "C" int rk_guide_mix(const PtArgs *a, const void *grid16, const void *w16
 }
 
 extern "C" int rk_guide_targets(void *rec, const unsigned *counts, const void *paths,
-                                float *wsum, float *dense, unsigned cap, void *stream) {
+                                float *wsum, float *dense, unsigned cap, unsigned mode,
+                                void *stream) {
   hipLaunchKernelGGL(k_guide_targets, dim3(256), dim3(256), 0, (hipStream_t)stream,
-                     (GdRecord *)rec, counts, (const PtPath *)paths, wsum, dense, cap);
+                     (GdRecord *)rec, counts, (const PtPath *)paths, wsum, dense, cap, mode);
   return (int)hipGetLastError();
 }
 

@@ -160,6 +160,7 @@ __global__ void k_pt_gen(PtArgs a, const PtView *__restrict__ views,
   p.depth = 0;
   p.pdf_prev = 0.0f;
   p.rec = 0xffffffffu;
+  p.rec_prev = 0xffffffffu;
   paths[i] = p;
   store_ray(rays, i, ox, oy, oz, kBig, tx * il, ty * il, tz * il);
   qpath[i] = i;
@@ -369,6 +370,7 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
                   rec.thr = 0.2126f * p.thr[0] + 0.7152f * p.thr[1] + 0.0722f * p.thr[2];
                   rec.path = pi;
                   rec.target = 0.0f;
+                  rec.l1 = 0.0f;
                 }
               }
             }
@@ -379,6 +381,7 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
     }
     const uint32_t rs = wave_append(&counts[PT_CNT_TRAIN], want_rec);
     if (valid) {
+      p.rec_prev = p.rec;  // the previous vertex's record (its target gathers this vertex's NEE)
       p.rec = 0xffffffffu;
       if (want_rec && rs < a.train_cap) {
         train[rs] = rec;
@@ -418,8 +421,10 @@ __global__ void k_pt_shadow(PtPath *__restrict__ paths, const uint32_t *__restri
     p.L[2] += c2;
     // direct light of this vertex is not incident radiance along the
     // continuation: keep it out of the training target
-    if (rec && p.rec != 0xffffffffu)
-      rec[p.rec].lbefore += 0.2126f * c0 + 0.7152f * c1 + 0.0722f * c2;
+    const float lum = 0.2126f * c0 + 0.7152f * c1 + 0.0722f * c2;
+    if (rec && p.rec != 0xffffffffu) rec[p.rec].lbefore += lum;
+    // ... and is the previous vertex's next-vertex direct light
+    if (rec && p.rec_prev != 0xffffffffu) rec[p.rec_prev].l1 += lum;
   }
 }
 

@@ -30,6 +30,7 @@ struct PtArgs {
 	uint32_t  train_cap;      /* training records per batch */
 	uint32_t  guide_depth;    /* vertices at depth < guide_depth are guided */
 	uint32_t  guide_share;    /* depth 0: one proposal per this many paths */
+	uint32_t  guide_target;   /* training target: 0 path radiance, 1 next-vertex direct */
 };
 
 /* Path state, 64 bytes. */
@@ -41,7 +42,8 @@ struct PtPath {
 	uint32_t  depth;
 	float     pdf_prev;   /* pdf of the last sampled direction */
 	uint32_t  rec;        /* training record of the current vertex, or ~0 */
-	uint32_t  pad[5];
+	uint32_t  rec_prev;   /* training record of the previous vertex, or ~0 */
+	uint32_t  pad[4];
 };
 
 /* Emissive triangle in world space, 64 bytes; pick = probability of
@@ -69,8 +71,11 @@ struct PtLight {
 #define GD_DEFAULT_CHUNK 32768u
 /* records drawn in proportion to their target (1) or uniformly (0) */
 #define GD_DEFAULT_IMPORTANCE 1
-/* guided vertices: depth < GD_DEFAULT_DEPTH */
-#define GD_DEFAULT_DEPTH 2
+/* guided vertices: depth < GD_DEFAULT_DEPTH (1: primary hits only) */
+#define GD_DEFAULT_DEPTH 1
+/* training target: 0 = all radiance the path gathers after the vertex,
+   1 = the next vertex's direct light */
+#define GD_DEFAULT_TARGET 0
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 
@@ -107,6 +112,8 @@ struct GdRecord {
 	float     thr;        /* throughput luminance after the continuation */
 	uint32_t  path;
 	float     target;     /* incident radiance estimate, filled after the frame */
+	float     l1;         /* luminance of the next vertex's direct light */
+	uint32_t  pad;
 };
 
 #define PT_CNT_CUR     0

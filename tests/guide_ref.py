@@ -25,7 +25,8 @@ OFS_B4 = OFS_W4 + GOUT * HID
 NPARAM = OFS_B4 + GOUT
 
 RECORD = np.dtype([("pos", "<f4", 3), ("nrm", "<f4", 3), ("dir", "<f4", 3), ("pdf", "<f4"),
-                   ("lbefore", "<f4"), ("thr", "<f4"), ("path", "<u4"), ("target", "<f4")])
+                   ("lbefore", "<f4"), ("thr", "<f4"), ("path", "<u4"), ("target", "<f4"),
+                   ("l1", "<f4"), ("pad", "<u4")])
 
 
 def resolutions():
