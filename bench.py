@@ -282,11 +282,15 @@ def bench_guide(a):
             tag=""):
         L.rend_clraccum(scene.rd)
         if guided:
-            check(L.rend_hip_guide_enable(scene.rd, *cfg, 7), "guide")
+            # a fresh network; the first frames train harder
+            warm = (cfg[0], cfg[1], a.guide_warm_steps, cfg[3]) if a.guide_warm_frames else cfg
+            check(L.rend_hip_guide_enable(scene.rd, *warm, 7), "guide")
         else:
             L.rend_hip_guide_disable(scene.rd)
         curve, el, samples, k = [], 0.0, sample0, 0
         while k < nframes:
+            if guided and a.guide_warm_frames and k == a.guide_warm_frames:
+                check(L.rend_hip_guide_set(scene.rd, cfg[0], cfg[2]), "guide steps")
             t = time.perf_counter()
             L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
             nr = C.c_uint(0)
@@ -333,7 +337,8 @@ def bench_guide(a):
                     "curve": cp},
           "guided": {"seconds": round(tg, 3), "relmse": err_g, "reached": reached,
                      "curve": cg, "adam_steps": int(info[2])},
-          "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_batch": cfg[2],
+          "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
+                    "warm_frames": a.guide_warm_frames, "warm_steps_per_frame": a.guide_warm_steps,
                     "lr": cfg[3]}},
          higher=False)
     scene.close()
@@ -364,6 +369,8 @@ def main():
     ap.add_argument("--guide-stride", type=int, default=2)
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
+    ap.add_argument("--guide-warm-frames", type=int, default=8)
+    ap.add_argument("--guide-warm-steps", type=int, default=32)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     a = ap.parse_args()
     if a.gpus != 1:
