@@ -369,7 +369,7 @@ def main():
     ap.add_argument("--guide-stride", type=int, default=2)
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
-    ap.add_argument("--guide-warm-frames", type=int, default=8)
+    ap.add_argument("--guide-warm-frames", type=int, default=0)
     ap.add_argument("--guide-warm-steps", type=int, default=32)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     a = ap.parse_args()
