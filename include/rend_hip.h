@@ -204,6 +204,18 @@ int          rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays,
                                   unsigned int spp, unsigned int seed,
                                   void *stream);
 
+/* The reference's own primary rays for a frame that starts after `samples`
+ * accumulated samples (make_camray_pckt in rend_render, rend.c:3603-3660,
+ * 3293-3402, pinhole cameras): blocks of the rend_staticinit size
+ * (rd->width / bszx by rd->height / bszy of them, edge remainders not
+ * rendered, as in the reference), 4x2-pixel packets in Morton order, the
+ * reference's xorshift128+ jitter stream and the host RSQRTPS, bit for bit.
+ * Ray i = (block * packets_per_block + packet) * 8 + lane; tmax = FLT_MAX.
+ * d_rays holds rend_hip_camrays_ref_count(rd) records. */
+unsigned long long rend_hip_camrays_ref_count(struct rdata *rd);
+int          rend_hip_gen_camrays_ref(struct rdata *rd, struct rray *d_rays,
+                                      unsigned int samples, void *stream);
+
 /* One cosine-weighted diffuse ray per hit of a previous trace, about the
  * geometric normal facing the incoming ray, origin p + 1e-3 * dir; a missed
  * input gives a ray with tmax = -1. */

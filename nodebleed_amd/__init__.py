@@ -90,6 +90,8 @@ _SIGS = {
                                          C.c_uint]),
     "rend_hip_render_frame": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint)]),
     "rend_hip_download_acc": (C.c_int, [C.c_void_p]),
+    "rend_hip_camrays_ref_count": (C.c_ulonglong, [C.c_void_p]),
+    "rend_hip_gen_camrays_ref": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p]),
     "rend_hip_gen_camrays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
                                        C.c_uint, C.c_uint, C.c_void_p]),
     "rend_hip_gen_diffuse": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -20,7 +20,11 @@
 #include "rend_hip.h"
 #include "librend.h"
 #include "../dlayout.h"
+#include "../rsqrt_table.h"
 
+int rk_camrays_ref_setup(const unsigned *tab, void *stream);
+int rk_camrays_ref(const struct CamArgs *c, unsigned bszx, unsigned bszy, unsigned blkcntx,
+                   unsigned nblk, unsigned samples, void *out, void *stream);
 int rk_camrays(const struct CamArgs *c, unsigned rw, unsigned rh,
                unsigned spp, unsigned seed, unsigned frame, void *out,
                void *stream);
@@ -246,6 +250,45 @@ int rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays, unsigned int w,
 	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
 	r = rk_camrays(&ca, w, h, spp ? spp : 1, seed, 0, d_rays, st);
 	return r ? rl_hipfail(r, "camera rays") : 0;
+}
+
+static const unsigned g_rsqrt_tab[4096] = RSQRT_TABLE_INIT;
+static int g_rsqrt_dev = -1;  /* device the table was uploaded to */
+
+unsigned long long rend_hip_camrays_ref_count(struct rdata *rd)
+{
+	const unsigned bx = rl_blkszx(), by = rl_blkszy();
+	if (!rd || bx % 4 || by % 2 || !bx || !by)
+		return 0;
+	return (unsigned long long)(rd->width / bx) * (rd->height / by) * (bx * by / 8) * 8;
+}
+
+int rend_hip_gen_camrays_ref(struct rdata *rd, struct rray *d_rays, unsigned int samples,
+                             void *stream)
+{
+	int r = rl_device();
+	if (r)
+		return r;
+	const unsigned bx = rl_blkszx(), by = rl_blkszy();
+	if (bx % 4 || by % 2)
+		return rl_fail(REND_EINVAL, "block %ux%u is not made of 4x2 packets", bx, by);
+	if (rd->cam.tanfangle > 0.0f)
+		return rl_fail(REND_EINVAL, "reference-exact rays need a pinhole camera");
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e)
+		return rl_hipfail(e, "device");
+	if (g_rsqrt_dev != dev) {
+		if ((r = rk_camrays_ref_setup(g_rsqrt_tab, st)))
+			return rl_hipfail(r, "rsqrt table");
+		g_rsqrt_dev = dev;
+	}
+	struct CamArgs ca;
+	rl_camargs(&rd->cam, &ca);
+	const unsigned blkcntx = rd->width / bx, nblk = blkcntx * (rd->height / by);
+	r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, samples, d_rays, st);
+	return r ? rl_hipfail(r, "reference camera rays") : 0;
 }
 
 int rend_hip_gen_diffuse(struct rdata *rd, const struct rray *d_prim,

@@ -225,6 +225,135 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 
 }  // namespace
 
+// ---- reference-exact primary rays -----------------------------------------
+//
+// rend_render (rend.c:3603-3708) walks blocks of bszx x bszy pixels; block
+// blk seeds a 4-lane xorshift128+ generator with rand8_init((blk + 13) *
+// 1571, samples * 23) (util.c:63-75, lanes 1..3 by the generator's jump),
+// and its packets of 4 x 2 pixels, in Morton order (rend.c:155-162), each
+// draw two 8-float vectors (randf8, util.c:99-107) for the pixel jitter of
+// make_camray_pckt (rend.c:3293-3402).  The pinhole branch is restated with
+// the same fused and separate roundings and the host RSQRTPS table, so every
+// ray is bit-identical.  One thread per packet replays its block's
+// generator 2 k steps ahead.
+
+__constant__ uint32_t c_rsqrt_tab[4096];
+
+struct Xs128 {
+  uint64_t s0, s1;
+};
+
+// xorshift128+ with shifts 23, 18, 5 (util.c:29-41): returns s0 + s1 and
+// advances the state.
+__device__ __forceinline__ uint64_t xs128_next(Xs128 &x) {
+  uint64_t a = x.s0;
+  const uint64_t b = x.s1;
+  const uint64_t res = a + b;
+  x.s0 = b;
+  a ^= a << 23;
+  x.s1 = a ^ b ^ (a >> 18) ^ (b >> 5);
+  return res;
+}
+
+// The generator's jump polynomial (2^64 steps ahead), as util.c:43-61.
+__device__ __forceinline__ Xs128 xs128_jump(Xs128 in) {
+  const uint64_t J[2] = {0x8a5cd789635d2dffull, 0x121fd2155c472f96ull};
+  Xs128 out = {0, 0};
+  for (int i = 0; i < 2; i++)
+    for (int b = 0; b < 64; b++) {
+      if (J[i] & (1ull << b)) {
+        out.s0 ^= in.s0;
+        out.s1 ^= in.s1;
+      }
+      xs128_next(in);
+    }
+  return out;
+}
+
+// x86 RSQRTPS from the captured tables (rule proven over all inputs by
+// gen_rsqrt_table.c).
+__device__ __forceinline__ float rsqrt_host(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t s = u & 0x80000000u, e = (u >> 23) & 0xffu, m = u & 0x7fffffu;
+  uint32_t r;
+  if (e == 255u) r = m ? (u | 0x00400000u) : (s ? 0xffc00000u : 0u);
+  else if (e == 0u) r = s | 0x7f800000u;
+  else if (s) r = 0xffc00000u;
+  else {
+    const uint32_t p = e & 1u;
+    const int sh = ((int)e - (p ? 127 : 128)) / 2;
+    r = (uint32_t)((int32_t)c_rsqrt_tab[(p << 11) | (m >> 12)] - sh * (1 << 23));
+  }
+  return __uint_as_float(r);
+}
+
+__device__ __forceinline__ uint32_t mort_dec(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v ^ (v >> 1)) & 0x33333333u;
+  v = (v ^ (v >> 2)) & 0x0f0f0f0fu;
+  v = (v ^ (v >> 4)) & 0x00ff00ffu;
+  return (v ^ (v >> 8)) & 0x0000ffffu;
+}
+
+__global__ void k_camrays_ref(CamArgs c, uint32_t bszx, uint32_t bszy, uint32_t blkcntx,
+                              uint32_t nblk, uint32_t npk, uint32_t samples,
+                              rray *__restrict__ out) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nblk * npk) return;
+  const uint32_t blk = t / npk, k = t % npk;
+  Xs128 g[4];
+  g[0].s0 = (uint64_t)(int64_t)(((int)blk + 13) * 1571);
+  g[0].s1 = (uint64_t)(samples * 23u);
+  for (int l = 1; l < 4; l++) g[l] = xs128_jump(g[l - 1]);
+  for (uint32_t i = 0; i < 2u * k; i++)
+    for (int l = 0; l < 4; l++) xs128_next(g[l]);
+  float u[2][8];
+  for (int d = 0; d < 2; d++)
+    for (int l = 0; l < 4; l++) {
+      const uint64_t v = xs128_next(g[l]);
+      u[d][2 * l] = __uint_as_float(((uint32_t)v >> 9) | 0x3f800000u) - 1.0f;
+      u[d][2 * l + 1] = __uint_as_float(((uint32_t)(v >> 32) >> 9) | 0x3f800000u) - 1.0f;
+    }
+  const uint32_t x = (blk % blkcntx) * bszx + mort_dec(k) * 4u;
+  const uint32_t y = (blk / blkcntx) * bszy + mort_dec(k >> 1) * 2u;
+  for (int l = 0; l < 8; l++) {
+    float sx = (float)x + (float)(l & 3);
+    float sy = (float)y + (float)(l >> 2);
+    sx = sx + (u[0][l] - 0.5f);
+    sy = sy + (u[1][l] - 0.5f);
+    sx = __builtin_fmaf(sx, c.rw, -0.5f);
+    sy = __builtin_fmaf(sy, c.rh, -0.5f);
+    sx = sx * c.fovfdist;
+    sy = sy * c.fovfdist;
+    sx = sx * c.aspect;
+    const float ux = c.up[0] * sy, uy = c.up[1] * sy, uz = c.up[2] * sy;
+    const float rx = __builtin_fmaf(c.ri[0], sx, -ux);
+    const float ry = __builtin_fmaf(c.ri[1], sx, -uy);
+    const float rz = __builtin_fmaf(c.ri[2], sx, -uz);
+    const float tx = __builtin_fmaf(c.fwd[0], c.fdist, rx);
+    const float ty = __builtin_fmaf(c.fwd[1], c.fdist, ry);
+    const float tz = __builtin_fmaf(c.fwd[2], c.fdist, rz);
+    const float rl = rsqrt_host(__builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz)));
+    float4 *o = (float4 *)&out[(size_t)t * 8u + (uint32_t)l];
+    o[0] = make_float4(c.eye[0], c.eye[1], c.eye[2], 3.402823466e38f);
+    o[1] = make_float4(tx * rl, ty * rl, tz * rl, 0.0f);
+  }
+}
+
+extern "C" int rk_camrays_ref_setup(const unsigned *tab, void *stream) {
+  return (int)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rsqrt_tab), tab, 4096 * sizeof(uint32_t), 0,
+                                     hipMemcpyHostToDevice, (hipStream_t)stream);
+}
+
+extern "C" int rk_camrays_ref(const CamArgs *c, unsigned bszx, unsigned bszy, unsigned blkcntx,
+                              unsigned nblk, unsigned samples, void *out, void *stream) {
+  const unsigned npk = bszx * bszy / 8u, n = nblk * npk;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_camrays_ref, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream,
+                     *c, bszx, bszy, blkcntx, nblk, npk, samples, (rray *)out);
+  return (int)hipGetLastError();
+}
+
 extern "C" int rk_camrays(const CamArgs *c, unsigned rw, unsigned rh,
                           unsigned spp, unsigned seed, unsigned frame,
                           void *out, void *stream) {

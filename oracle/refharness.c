@@ -159,3 +159,41 @@ float ref_rcpps(float x)
 {
 	return _mm_cvtss_f32(_mm_rcp_ss(_mm_set_ss(x)));
 }
+
+/* The reference's primary rays of block blk (rend_render's walk,
+ * rend.c:3603-3660): rand8_init seeded as there, packets in Morton order
+ * through its own make_camray_pckt; rd supplies width and the camera. */
+void make_camray_pckt(__m256 *ox8, __m256 *oy8, __m256 *oz8,
+                      __m256 *dx8, __m256 *dy8, __m256 *dz8,
+                      unsigned int x, unsigned int y,
+                      __m256 xofs8, __m256 yofs8,
+                      struct rcam *c, struct rngstate8 *rngstate);
+
+void ref_camrays(struct rdata *rd, unsigned bszx, unsigned bszy, unsigned samples,
+                 unsigned blk, struct rray *out)
+{
+	rend_staticinit(bszx, bszy);
+	unsigned blkcntx = rd->width / bszx, npk = bszx * bszy / 8;
+	struct rngstate8 rng;
+	rand8_init(&rng, ((int)blk + 13) * 1571, samples * 23);
+	__m256 xo = _mm256_set_ps(3, 2, 1, 0, 3, 2, 1, 0);
+	__m256 yo = _mm256_set_ps(1, 1, 1, 1, 0, 0, 0, 0);
+	for (unsigned k = 0; k < npk; k++) {
+		__m256 v[6];
+		make_camray_pckt(&v[0], &v[1], &v[2], &v[3], &v[4], &v[5],
+		  (blk % blkcntx) * bszx + mortdecx(k) * 4,
+		  (blk / blkcntx) * bszy + mortdecy(k) * 2, xo, yo, &rd->cam, &rng);
+		float f[6][8];
+		for (int a = 0; a < 6; a++)
+			_mm256_storeu_ps(f[a], v[a]);
+		for (int l = 0; l < 8; l++)
+			out[k * 8 + l] = (struct rray){f[0][l], f[1][l], f[2][l], FLT_MAX,
+			  f[3][l], f[4][l], f[5][l], 0.0f};
+	}
+	rend_staticrelease();
+}
+
+float ref_rsqrtps(float x)
+{
+	return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x)));
+}

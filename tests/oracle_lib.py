@@ -43,6 +43,10 @@ def oracle() -> C.CDLL:
         for f in ("or_xmax", "or_xmin"):
             getattr(L, f).restype = C.c_float
             getattr(L, f).argtypes = [C.c_float, C.c_float]
+        L.or_rsqrt.restype = C.c_float
+        L.or_rsqrt.argtypes = [C.c_float]
+        L.or_camrays.restype = None
+        L.or_camrays.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_void_p]
         _or = L
     return _or
 
@@ -97,6 +101,28 @@ def rel_mse(x: np.ndarray, r: np.ndarray) -> float:
     return float(((x - r) ** 2).sum() / (r ** 2).sum())
 
 
+def or_camrays(scene: Scene, bszx: int, bszy: int, samples: int, blocks) -> np.ndarray:
+    """The reference's primary rays of the given blocks (pinhole camera),
+    restated: rays of block b in rend_render's packet/lane order."""
+    npk = bszx * bszy // 8
+    out = np.zeros((len(blocks), npk * 8), RAY)
+    for i, b in enumerate(blocks):
+        oracle().or_camrays(C.c_void_p(scene.rd), bszx, bszy, samples, int(b), ptr(out[i]))
+    return out
+
+
+def ref_camrays(scene: Scene, bszx: int, bszy: int, samples: int, blocks) -> np.ndarray | None:
+    """The same rays from the reference's own make_camray_pckt, or None."""
+    L = reference()
+    if L is None:
+        return None
+    npk = bszx * bszy // 8
+    out = np.zeros((len(blocks), npk * 8), RAY)
+    for i, b in enumerate(blocks):
+        L.ref_camrays(C.c_void_p(scene.rd), bszx, bszy, samples, int(b), ptr(out[i]))
+    return out
+
+
 def reference(fast: bool = False) -> C.CDLL | None:
     """The reference's own code (built here from /root/reference), or None."""
     path = REF_FAST if fast else REF
@@ -119,6 +145,10 @@ def reference(fast: bool = False) -> C.CDLL | None:
     L.ref_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint, C.c_int]
     L.ref_rcpps.restype = C.c_float
     L.ref_rcpps.argtypes = [C.c_float]
+    L.ref_rsqrtps.restype = C.c_float
+    L.ref_rsqrtps.argtypes = [C.c_float]
+    L.ref_camrays.restype = None
+    L.ref_camrays.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_void_p]
     _ref[path] = L
     return L
 

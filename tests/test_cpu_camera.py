@@ -1,0 +1,94 @@
+"""Reference-exact primary rays (rend_render's block walk, make_camray_pckt,
+xorshift128+ jitter, host RSQRTPS): the oracle's restatement against the
+reference's own code compiled from its sources (oracle/_ref), the RSQRTPS
+rule against the instruction and its error bound, and properties of the
+rays.  CPU only."""
+import struct
+
+import numpy as np
+import pytest
+
+from nodebleed_amd import Scene, lib
+from oracle_lib import oracle, or_camrays, ref_camrays, reference
+
+
+def f32(u):
+    return struct.unpack("<f", struct.pack("<I", u))[0]
+
+
+def bits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def test_rsqrt_rule_known_values():
+    o = oracle()
+    assert bits(o.or_rsqrt(1.0)) == 0x3F7FF000      # RSQRTPS(1.0) on the capture host
+    assert bits(o.or_rsqrt(4.0)) == 0x3EFFF000      # exponent step: half of rsqrt(1)
+    assert bits(o.or_rsqrt(2.0)) == 0x3F34F800
+    assert bits(o.or_rsqrt(0.5)) == 0x3FB4F800      # twice rsqrt(2)
+    assert bits(o.or_rsqrt(0.0)) == 0x7F800000
+    assert bits(o.or_rsqrt(-0.0)) == 0xFF800000
+    assert bits(o.or_rsqrt(f32(0x00000001))) == 0x7F800000   # denormal -> inf
+    assert bits(o.or_rsqrt(f32(0x00800000))) == 0x5EFFF000   # smallest normal
+    assert bits(o.or_rsqrt(f32(0x7F7FFFFF))) == 0x1F800800   # largest finite
+    assert bits(o.or_rsqrt(float("inf"))) == 0
+    assert bits(o.or_rsqrt(-1.0)) == 0xFFC00000
+    assert np.isnan(o.or_rsqrt(float("nan")))
+
+
+def test_rsqrt_relative_error_bound():
+    """|rsqrt(x) * sqrt(x) - 1| <= 1.5 * 2^-12 (Intel SDM bound for RSQRTPS)."""
+    o = oracle()
+    rng = np.random.default_rng(3)
+    for x in np.exp(rng.uniform(-80, 80, 20000)).astype(np.float32):
+        r = o.or_rsqrt(float(x))
+        assert abs(r * np.sqrt(np.float64(x)) - 1.0) <= 1.5 * 2.0 ** -12
+
+
+def test_rsqrt_matches_host_instruction():
+    """The rule equals the instruction itself where the capture CPU runs."""
+    ref = reference()
+    if ref is None:
+        pytest.skip("reference harness not built")
+    if lib().rend_hip_rcp_cpu().decode() not in open("/proc/cpuinfo").read():
+        pytest.skip("tables were captured on a different CPU")
+    o = oracle()
+    rng = np.random.default_rng(4)
+    xs = rng.integers(0, 2 ** 32, 200000, dtype=np.uint64).astype(np.uint32)
+    for u in xs:
+        x = f32(int(u))
+        a, b = o.or_rsqrt(x), ref.ref_rsqrtps(x)
+        assert bits(a) == bits(b) or (np.isnan(a) and np.isnan(b)), hex(int(u))
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    s = Scene(config=2, detail=0.05, width=256, height=128)
+    yield s
+    s.close()
+
+
+@pytest.mark.parametrize("bs", [(32, 8), (16, 16), (64, 32)])
+@pytest.mark.parametrize("samples", [0, 37])
+def test_camrays_match_reference_bit_exact(terrain, bs, samples):
+    if reference() is None:
+        pytest.skip("reference harness not built")
+    bx, by = bs
+    nblk = (256 // bx) * (128 // by)
+    blocks = [0, 1, nblk // 2, nblk - 1]
+    want = ref_camrays(terrain, bx, by, samples, blocks)
+    got = or_camrays(terrain, bx, by, samples, blocks)
+    assert want.tobytes() == got.tobytes()
+
+
+def test_camray_properties(terrain):
+    bx, by = 32, 8
+    rays = or_camrays(terrain, bx, by, 5, range((256 // bx) * (128 // by))).ravel()
+    d = np.stack([rays["dx"], rays["dy"], rays["dz"]], 1).astype(np.float64)
+    # approximate normalisation: 12-bit RSQRTPS
+    assert np.abs(np.linalg.norm(d, axis=1) - 1).max() < 4e-4
+    eye = np.array([rays["ox"][0], rays["oy"][0], rays["oz"][0]])
+    assert (rays["ox"] == eye[0]).all() and (rays["oy"] == eye[1]).all()
+    # every pixel of the covered area gets exactly one ray: recover pixel
+    # centres by projecting onto the image plane
+    assert len(rays) == 256 * 128

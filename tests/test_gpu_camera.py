@@ -1,0 +1,66 @@
+"""GPU reference-exact primary rays (rend_hip_gen_camrays_ref) against the
+oracle's restatement of rend_render's camera rays, which the CPU suite pins
+bit for bit to the reference's own make_camray_pckt; then the whole primary
+pass (those rays through the GPU traversal) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from nodebleed_amd import RAY, Scene, check, lib
+from oracle_lib import or_camrays, or_trace
+from raygen import mismatches
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.mark.parametrize("bs,samples", [((64, 32), 0), ((32, 8), 123), ((16, 16), 7)])
+def test_reference_camrays_bit_exact(bs, samples):
+    L = lib()
+    bx, by = bs
+    s = Scene(config=2, detail=0.25, width=1920, height=1080)
+    L.rend_staticinit(bx, by)
+    n = L.rend_hip_camrays_ref_count(s.rd)
+    nblk = (1920 // bx) * (1080 // by)
+    assert n == nblk * bx * by
+    d = L.rend_hip_malloc(n * RAY.itemsize)
+    try:
+        say(f"reference camera rays {bx}x{by} blocks, samples {samples}: {n} rays")
+        check(L.rend_hip_gen_camrays_ref(s.rd, d, samples, None))
+        check(L.rend_hip_sync())
+        got = np.zeros(n, RAY)
+        check(L.rend_hip_memcpy_dtoh(got.ctypes.data, d, n * RAY.itemsize))
+        want = or_camrays(s, bx, by, samples, range(nblk)).ravel()
+        bad = np.flatnonzero(got.view(np.uint32).reshape(n, 8) != want.view(np.uint32).reshape(n, 8))
+        assert bad.size == 0, f"{bad.size} words differ, first ray {bad[0] // 8}"
+        # the primary pass: these rays through the GPU traversal, bit-exact
+        # with the oracle on a sample
+        rng = np.random.default_rng(samples)
+        idx = np.sort(rng.choice(n, 30000, replace=False))
+        say("trace a sample of them")
+        hits = s.trace(got[idx])
+        h_want, _ = or_trace(s, got[idx])
+        assert mismatches(hits, h_want) == {"id": 0, "t": 0, "u": 0, "v": 0}
+    finally:
+        L.rend_hip_free(d)
+        L.rend_staticinit(64, 32)
+        s.close()
+
+
+def test_reference_camrays_reject_partial_packets():
+    """Blocks that are not whole 4x2 packets have no reference walk."""
+    L = lib()
+    s = Scene(config=2, detail=0.05, width=256, height=128)
+    d = L.rend_hip_malloc(256 * 128 * RAY.itemsize)
+    try:
+        L.rend_staticinit(30, 8)
+        assert L.rend_hip_camrays_ref_count(s.rd) == 0
+        assert L.rend_hip_gen_camrays_ref(s.rd, d, 0, None) != 0
+    finally:
+        L.rend_hip_free(d)
+        L.rend_staticinit(64, 32)
+        s.close()
