@@ -285,6 +285,7 @@ def bench_guide(a):
             # a fresh network; the first frames train harder
             warm = (cfg[0], cfg[1], a.guide_warm_steps, cfg[3]) if a.guide_warm_frames else cfg
             check(L.rend_hip_guide_enable(scene.rd, *warm, 7), "guide")
+            check(L.rend_hip_guide_training(scene.rd, a.guide_chunk, cfg[3], cfg[3], 1), "schedule")
         else:
             L.rend_hip_guide_disable(scene.rd)
         curve, el, samples, k = [], 0.0, sample0, 0
@@ -339,6 +340,7 @@ def bench_guide(a):
                      "curve": cg, "adam_steps": int(info[2])},
           "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
                     "warm_frames": a.guide_warm_frames, "warm_steps_per_frame": a.guide_warm_steps,
+                    "records_per_step": a.guide_chunk,
                     "lr": cfg[3]}},
          higher=False)
     scene.close()
@@ -370,6 +372,8 @@ def main():
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
     ap.add_argument("--guide-warm-frames", type=int, default=0)
+    ap.add_argument("--guide-chunk", type=int, default=16384,
+                    help="training records per Adam step")
     ap.add_argument("--guide-warm-steps", type=int, default=32)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     a = ap.parse_args()
