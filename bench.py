@@ -364,15 +364,15 @@ def main():
                     help="trace: closest-hit microbench (the headline line); pt: "
                          "instanced path tracing; views: 32 uneven cameras in one batched "
                          "launch; guide: equal-time guided vs plain path tracing")
-    ap.add_argument("--guide-frames", type=int, default=64)
-    ap.add_argument("--guide-ref-frames", type=int, default=1024)
+    ap.add_argument("--guide-frames", type=int, default=256)
+    ap.add_argument("--guide-ref-frames", type=int, default=4096)
     ap.add_argument("--guide-spf", type=int, default=16, help="samples per pixel per frame")
     ap.add_argument("--guide-mix", type=float, default=0.5)
     ap.add_argument("--guide-stride", type=int, default=2)
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
     ap.add_argument("--guide-warm-frames", type=int, default=0)
-    ap.add_argument("--guide-chunk", type=int, default=16384,
+    ap.add_argument("--guide-chunk", type=int, default=4096,
                     help="training records per Adam step")
     ap.add_argument("--guide-warm-steps", type=int, default=32)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")

@@ -142,7 +142,7 @@ int          rend_hip_pt_graphs(int on);
  * point; the indirect direction is drawn from mix * mixture + (1 - mix) *
  * cosine BSDF (one-sample MIS, unbiased for 0 <= mix < 1).  The network is
  * trained online from the vertices of every train_stride-th path: `steps`
- * Adam steps of 16K records per frame, spread over its wavefront batches,
+ * Adam steps of 4K records per frame, spread over its wavefront batches,
  * records drawn in proportion to their radiance targets, learning rate lr
  * for the hash grid and the MLP.  The proposal is used from the first
  * trained batch on. */
@@ -152,7 +152,7 @@ int          rend_hip_guide_enable(struct rdata *rd, float mix,
 int          rend_hip_guide_disable(struct rdata *rd);
 /* Changes mix and training steps (0 = frozen network) while enabled. */
 int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps);
-/* Training schedule while enabled: records per Adam step (default 16K),
+/* Training schedule while enabled: records per Adam step (default 4K),
  * learning rates of the hash grid and of the MLP, and whether records are
  * drawn in proportion to their target (importance != 0) or uniformly. */
 int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,

@@ -68,7 +68,7 @@ struct PtLight {
 #define GD_OUT     64
 #define GD_LOBES   8
 /* training records per optimiser step unless set otherwise */
-#define GD_DEFAULT_CHUNK 16384u
+#define GD_DEFAULT_CHUNK 4096u
 /* records drawn in proportion to their target (1) or uniformly (0) */
 #define GD_DEFAULT_IMPORTANCE 1
 /* guided vertices: depth < GD_DEFAULT_DEPTH (1: primary hits only) */

@@ -198,6 +198,7 @@ def test_training_learns_a_known_lobe(guided_scene):
     s = guided_scene
     L = lib()
     check(L.rend_hip_guide_enable(s.rd, 0.5, 2, 2, 1e-2, 21), "guide re-init")
+    check(L.rend_hip_guide_training(s.rd, 32768, 1e-2, 1e-2, 1), "schedule")
     rng = np.random.default_rng(8)
     ab = s.aabbs()
     lo, hi = ab["min"].min(axis=0), ab["max"].max(axis=0)
