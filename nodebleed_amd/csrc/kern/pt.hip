@@ -13,9 +13,9 @@
 // One pass = every active path advances by one segment:
 //   k_pt_gen      camera rays for a batch of paths (views x pixels x spp)
 //   (trace)       closest hit of the ray queue   (trace.hip, count on device)
-//   k_pt_shade    emission, NEE shadow ray, next ray; compacts the next-ray
-//                 and shadow queues with wave ballot + mbcnt and one atomic
-//                 per wave
+//   k_pt_shade    emission, NEE shadow ray, next ray; compacts the next-ray,
+//                 shadow and training-record queues with wave ballots, mbcnt
+//                 and one atomic per queue per block
 //   (occluded)    any hit of the shadow queue
 //   k_pt_shadow   adds unoccluded NEE contributions to their paths
 //   k_pt_resolve  adds path radiance to the view accumulators, ARGB8 output
@@ -59,20 +59,6 @@ struct RTri {
 
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-}
-
-// Wave-aggregated append: one atomic per wave, slots in lane order.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want) {
-  const uint64_t m = __ballot(want);
-  if (m == 0) return 0xffffffffu;
-  const uint32_t lane = lane_id();
-  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader);
-  const uint32_t below = __builtin_amdgcn_mbcnt_hi(
-      (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-  return want ? base + below : 0xffffffffu;
 }
 
 // A direction component of exactly +-0 gives infinite 1/d and o/d; under the
@@ -230,7 +216,49 @@ __device__ __forceinline__ void sample_light(const PtLight *__restrict__ lights,
   pdf_area = pk / L.area;
 }
 
-__global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
+// Slots in the training-record, next-ray and shadow queues for one trip of
+// a 256-thread block: one atomic per queue per block (every wave of the
+// launch appends to the same three counters, so per-wave atomics queue up
+// on three addresses), slots in wave then lane order.
+struct BlockAppend {
+  uint32_t cnt[4][3];
+  uint32_t base[3];
+};
+
+__device__ __forceinline__ void block_append3(uint32_t *counts, bool w0, bool w1, bool w2,
+                                              uint32_t &s0, uint32_t &s1, uint32_t &s2,
+                                              BlockAppend &S) {
+  const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+  const uint64_t m0 = __ballot(w0), m1 = __ballot(w1), m2 = __ballot(w2);
+  if (lane == 0) {
+    S.cnt[wv][0] = (uint32_t)__popcll(m0);
+    S.cnt[wv][1] = (uint32_t)__popcll(m1);
+    S.cnt[wv][2] = (uint32_t)__popcll(m2);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3u) {
+    const uint32_t q = threadIdx.x;
+    const uint32_t tot = S.cnt[0][q] + S.cnt[1][q] + S.cnt[2][q] + S.cnt[3][q];
+    const uint32_t ctr = q == 0 ? PT_CNT_TRAIN : (q == 1 ? PT_CNT_NEXT : PT_CNT_SHADOW);
+    S.base[q] = tot ? atomicAdd(&counts[ctr], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t b0 = S.base[0], b1 = S.base[1], b2 = S.base[2];
+  for (uint32_t k = 0; k < wv; k++) {
+    b0 += S.cnt[k][0];
+    b1 += S.cnt[k][1];
+    b2 += S.cnt[k][2];
+  }
+  __syncthreads();  // S is reused by the next trip
+  const uint32_t l0 = (uint32_t)m0, h0 = (uint32_t)(m0 >> 32);
+  const uint32_t l1 = (uint32_t)m1, h1 = (uint32_t)(m1 >> 32);
+  const uint32_t l2 = (uint32_t)m2, h2 = (uint32_t)(m2 >> 32);
+  s0 = w0 ? b0 + __builtin_amdgcn_mbcnt_hi(h0, __builtin_amdgcn_mbcnt_lo(l0, 0u)) : 0xffffffffu;
+  s1 = w1 ? b1 + __builtin_amdgcn_mbcnt_hi(h1, __builtin_amdgcn_mbcnt_lo(l1, 0u)) : 0xffffffffu;
+  s2 = w2 ? b2 + __builtin_amdgcn_mbcnt_hi(h2, __builtin_amdgcn_mbcnt_lo(l2, 0u)) : 0xffffffffu;
+}
+
+__global__ __launch_bounds__(256) void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
                            const rray *__restrict__ rays, const rhit *__restrict__ hits,
                            const uint32_t *__restrict__ qpath, rray *__restrict__ nrays,
                            uint32_t *__restrict__ nqpath, rray *__restrict__ srays,
@@ -240,9 +268,10 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
                            const RMtl *__restrict__ mtls, const PtLight *__restrict__ lights,
                            const float *__restrict__ lcdf, const float *__restrict__ guide,
                            GdRecord *__restrict__ train, uint32_t depth) {
+  __shared__ BlockAppend s_app;
   const uint32_t n = counts[PT_CNT_CUR];
   const uint32_t stride = gridDim.x * blockDim.x;
-  // uniform trip count so every lane of a wave takes part in the appends
+  // uniform trip count so every thread of the block takes part in the appends
   const uint32_t trips = (n + stride - 1) / stride;
   for (uint32_t t = 0; t < trips; t++) {
     const uint32_t i = t * stride + blockIdx.x * blockDim.x + threadIdx.x;
@@ -379,22 +408,26 @@ __global__ void k_pt_shade(PtArgs a, PtPath *__restrict__ paths,
       }
       p.depth = depth + 1;
     }
-    const uint32_t rs = wave_append(&counts[PT_CNT_TRAIN], want_rec);
+    uint32_t rs, ns, ss;
+    block_append3(counts, want_rec, want_next, want_shadow, rs, ns, ss, s_app);
     if (valid) {
       p.rec_prev = p.rec;  // the previous vertex's record (its target gathers this vertex's NEE)
       p.rec = 0xffffffffu;
       if (want_rec && rs < a.train_cap) {
-        train[rs] = rec;
+        // four 16-byte stores (a 64-byte record per lane)
+        float4 *dst = (float4 *)&train[rs];
+        dst[0] = make_float4(rec.pos[0], rec.pos[1], rec.pos[2], rec.nrm[0]);
+        dst[1] = make_float4(rec.nrm[1], rec.nrm[2], rec.dir[0], rec.dir[1]);
+        dst[2] = make_float4(rec.dir[2], rec.pdf, rec.lbefore, rec.thr);
+        dst[3] = make_float4(__uint_as_float(rec.path), rec.target, rec.l1, 0.0f);
         p.rec = rs;
       }
       paths[pi] = p;
     }
-    const uint32_t ns = wave_append(&counts[PT_CNT_NEXT], want_next);
     if (want_next) {
       store_ray(nrays, ns, nro[0], nro[1], nro[2], kBig, nrd[0], nrd[1], nrd[2]);
       nqpath[ns] = pi;
     }
-    const uint32_t ss = wave_append(&counts[PT_CNT_SHADOW], want_shadow);
     if (want_shadow) {
       store_ray(srays, ss, sro[0], sro[1], sro[2], stmax, srd[0], srd[1], srd[2]);
       spath[ss] = pi;
