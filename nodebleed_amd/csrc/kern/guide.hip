@@ -229,8 +229,15 @@ __global__ void k_guide_targets(GdRecord *__restrict__ rec, const uint32_t *__re
     if (dense) dense[i] = r.target;
     acc += r.target;
   }
+  // one atomic per block (per-wave atomics would all queue on wsum)
+  __shared__ float s_part[4];
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
-  if ((threadIdx.x & 63u) == 0 && acc > 0.0f) atomicAdd(wsum, acc);
+  if ((threadIdx.x & 63u) == 0) s_part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float t = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (t > 0.0f) atomicAdd(wsum, t);
+  }
 }
 
 // Records drawn in proportion to their targets: inclusive prefix sums of the
