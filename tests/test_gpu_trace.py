@@ -2,7 +2,7 @@
 oracle (C restatement of the reference's intersect_tlas/intersect_any_tlas)
 on identical ray buffers.
 
-Bar: hit ids bit-exact and t/u/v bit-exact (the north-star bar is ids exact
+Bar: hit ids bit-exact and t/u/v bit-exact (the required bar is ids exact
 and t within 2 ulp; the kernel reproduces every operation, including the
 host RCPSS table, so the tests demand exact bits and report ulp if not).
 Visit counts (nodes, leaves, instances) must equal the oracle's, which also
