@@ -134,15 +134,20 @@ def test_gradient_matches_cpu(guided_scene):
         loss.value) + 1e-3
 
 
-def test_guided_rendering_is_unbiased():
+@pytest.mark.parametrize("target,depth,share", [(0, 1, 1), (1, 2, 0), (0, 5, 1)])
+def test_guided_rendering_is_unbiased(target, depth, share):
     """Plain and guided (trained online, one-sample MIS with the BSDF) images
-    converge to the same expectation.  The box scene keeps Monte Carlo noise
-    low enough for a tight bound: 3072 spp per image, 4x4 block means."""
+    converge to the same expectation, for either training target, guided
+    depth and with or without the per-pixel depth-0 proposal.  The box scene
+    keeps Monte Carlo noise low enough for a tight bound: 3072 spp per
+    image, 4x4 block means."""
     s = Scene(config=1, detail=0.25, width=64, height=64)
     L = lib()
     try:
         check(L.rend_hip_guide_enable(s.rd, 0.5, 2, 4, 1e-2, 11), "guide enable")
-        say("guided render 96 frames x 32 spp (box)")
+        check(L.rend_hip_guide_sampling(s.rd, depth, share), "sampling")
+        check(L.rend_hip_guide_target(s.rd, target), "target")
+        say(f"guided render 96 frames x 32 spp (box), target {target} depth {depth} share {share}")
         img_g, _ = render_spp(s, 96)
         info = (C.c_float * 4)()
         check(L.rend_hip_guide_info(s.rd, info))
