@@ -283,15 +283,14 @@ def bench_guide(a):
         L.rend_clraccum(scene.rd)
         if guided:
             # a fresh network; the first frames train harder
-            warm = (cfg[0], cfg[1], a.guide_warm_steps, cfg[3]) if a.guide_warm_frames else cfg
-            check(L.rend_hip_guide_enable(scene.rd, *warm, 7), "guide")
+            check(L.rend_hip_guide_enable(scene.rd, *cfg, 7), "guide")
             check(L.rend_hip_guide_training(scene.rd, a.guide_chunk, cfg[3], cfg[3], 1), "schedule")
+            check(L.rend_hip_guide_warmup(scene.rd, a.guide_warm_frames, a.guide_warm_steps),
+                  "warm-up")
         else:
             L.rend_hip_guide_disable(scene.rd)
         curve, el, samples, k = [], 0.0, sample0, 0
         while k < nframes:
-            if guided and a.guide_warm_frames and k == a.guide_warm_frames:
-                check(L.rend_hip_guide_set(scene.rd, cfg[0], cfg[2]), "guide steps")
             t = time.perf_counter()
             L.rs_rdata_setframe(scene.rd, 0, 0, samples, None)
             nr = C.c_uint(0)
@@ -318,7 +317,7 @@ def bench_guide(a):
     tp, plain, cp = run(a.guide_frames, False, ref=ref, check_every=every, tag="plain")
     err_p = rel_mse(plain, ref)
     log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.4f}")
-    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 2), stop_at=err_p,
+    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 4), stop_at=err_p,
                          tmax=3.0 * tp, tag="guided")
     err_g = rel_mse(guided, ref)
     info = (C.c_float * 4)()
@@ -368,13 +367,14 @@ def main():
     ap.add_argument("--guide-ref-frames", type=int, default=4096)
     ap.add_argument("--guide-spf", type=int, default=16, help="samples per pixel per frame")
     ap.add_argument("--guide-mix", type=float, default=0.5)
-    ap.add_argument("--guide-stride", type=int, default=2)
+    ap.add_argument("--guide-stride", type=int, default=32,
+                    help="record a training sample at every n-th path")
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2)
-    ap.add_argument("--guide-warm-frames", type=int, default=0)
+    ap.add_argument("--guide-warm-frames", type=int, default=16)
     ap.add_argument("--guide-chunk", type=int, default=4096,
                     help="training records per Adam step")
-    ap.add_argument("--guide-warm-steps", type=int, default=32)
+    ap.add_argument("--guide-warm-steps", type=int, default=64)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     a = ap.parse_args()
     if a.gpus != 1:

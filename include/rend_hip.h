@@ -165,6 +165,16 @@ int          rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int s
  * it (0, default) or only the next vertex's direct light (1: one bounce,
  * lower variance). */
 int          rend_hip_guide_target(struct rdata *rd, int next_vertex_direct);
+/* Training objective: the KL divergence from the normalised target to the
+ * vMF mixture (0, default) or to the density one-sample MIS actually samples,
+ * mix * vMF + (1 - mix) * cos/pi (1: lobes need not cover what the BSDF
+ * already samples well). */
+int          rend_hip_guide_objective(struct rdata *rd, int mis_density);
+/* Warm-up: the first `frames` frames after rend_hip_guide_enable train with
+ * `steps` Adam steps each (when that is more than the per-frame count), so
+ * the untrained proposal improves quickly (default 16 frames of 64 steps;
+ * frames = 0 turns it off). */
+int          rend_hip_guide_warmup(struct rdata *rd, unsigned int frames, unsigned int steps);
 /* out: last training loss, records of the last batch, Adam steps, enabled. */
 int          rend_hip_guide_info(struct rdata *rd, float out[4]);
 /* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),

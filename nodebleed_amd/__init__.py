@@ -119,6 +119,8 @@ _SIGS = {
     "rend_hip_guide_training": (C.c_int, [C.c_void_p, C.c_uint, C.c_float, C.c_float, C.c_int]),
     "rend_hip_guide_sampling": (C.c_int, [C.c_void_p, C.c_uint, C.c_int]),
     "rend_hip_guide_target": (C.c_int, [C.c_void_p, C.c_int]),
+    "rend_hip_guide_objective": (C.c_int, [C.c_void_p, C.c_int]),
+    "rend_hip_guide_warmup": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint]),
     "rend_hip_guide_cdf_host": (C.c_int, [C.c_void_p, C.c_uint, C.c_float, C.c_void_p]),
     "rend_hip_guide_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "rend_hip_guide_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),

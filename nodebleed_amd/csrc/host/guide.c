@@ -198,6 +198,10 @@ int rend_hip_guide_enable(struct rdata *rd, float mix, unsigned int train_stride
 	g->depth = GD_DEFAULT_DEPTH;
 	g->share = 1;
 	g->target = GD_DEFAULT_TARGET;
+	g->objective = GD_DEFAULT_OBJECTIVE;
+	g->warm_frames = GD_DEFAULT_WARM_FRAMES;
+	g->warm_steps = GD_DEFAULT_WARM_STEPS;
+	g->frames = 0;
 	g->enabled = 1;
 	hipError_t e = hipStreamSynchronize(G.stream);
 	return e ? rl_hipfail(e, "guide enable") : 0;
@@ -251,6 +255,46 @@ int rend_hip_guide_target(struct rdata *rd, int next_vertex_direct)
 	return 0;
 }
 
+int rend_hip_guide_objective(struct rdata *rd, int mis_density)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	s->guide->objective = mis_density != 0;
+	return 0;
+}
+
+int rend_hip_guide_warmup(struct rdata *rd, unsigned int frames, unsigned int steps)
+{
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->guide)
+		return rl_fail(REND_ESTATE, "guiding not enabled");
+	if (steps > 4096)
+		return rl_fail(REND_EINVAL, "warm-up steps %u (<= 4096)", steps);
+	s->guide->warm_frames = frames;
+	s->guide->warm_steps = steps;
+	return 0;
+}
+
+/* Adam steps of the next frame: while training is on, the warm-up count for
+ * the first warm_frames frames after enable (when it is larger), the
+ * per-frame count after that. */
+unsigned rl_guide_frame_steps(struct rl_guide *g)
+{
+	if (!g->steps)
+		return 0;
+	const unsigned n = g->frames < g->warm_frames && g->warm_steps > g->steps ?
+	  g->warm_steps : g->steps;
+	g->frames++;
+	return n;
+}
+
+/* BSDF share of the density the training loss fits. */
+static float train_bsdf(const struct rl_guide *g)
+{
+	return g->objective ? 1.0f - g->mix_prob : 0.0f;
+}
+
 int rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share)
 {
 	struct rl_scene *s = rl_scene(rd);
@@ -292,6 +336,7 @@ void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 	a->train_cap = (unsigned)(cap / g->stride + 64);
 	a->guide_depth = g->depth;
 	a->guide_share = 1;
+	a->train_bsdf = train_bsdf(g);
 }
 
 /* Per-batch buffers: mixtures per queued path and training records (as
@@ -402,6 +447,7 @@ static void test_args(struct rl_guide *g, struct PtArgs *a)
 	memset(a, 0, sizeof(*a));
 	memcpy(a->gmin, g->gmin, sizeof(a->gmin));
 	memcpy(a->ginv, g->ginv, sizeof(a->ginv));
+	a->train_bsdf = train_bsdf(g);
 }
 
 int rend_hip_guide_forward_host(struct rdata *rd, const float *pn, unsigned int n,

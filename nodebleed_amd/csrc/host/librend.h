@@ -53,6 +53,9 @@ struct rl_guide {
 	unsigned  depth;                    /* vertices at depth < depth are guided */
 	int       share;                    /* one depth-0 proposal per pixel */
 	int       target;                   /* 1: train on next-vertex direct light */
+	int       objective;                /* 1: fit the MIS density, not the vMF alone */
+	unsigned  warm_frames, warm_steps;  /* Adam steps of the first frames after enable */
+	unsigned  frames;                   /* frames trained since enable */
 	int       importance;               /* draw records in proportion to target */
 	int       enabled;
 };
@@ -101,6 +104,7 @@ void             rl_pt_release(struct rl_scene *s);
 void             rl_guide_release(struct rl_scene *s);
 struct PtArgs;
 void             rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap);
+unsigned         rl_guide_frame_steps(struct rl_guide *g);
 int              rl_guide_buffers(struct rl_scene *s, size_t cap);
 int              rl_guide_eval(struct rl_scene *s, const struct PtArgs *a,
                                 const void *rays, const void *hits,

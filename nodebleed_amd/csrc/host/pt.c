@@ -285,7 +285,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 	void *rec = guided && a->train_stride ? s->guide->rec : NULL;
 	/* the frame's training steps are spread over its batches */
 	const unsigned long long nb = (total + cap - 1) / cap;
-	const unsigned steps = rec ? s->guide->steps : 0;
+	const unsigned steps = rec ? rl_guide_frame_steps(s->guide) : 0;
 	unsigned long long b = 0;
 	for (unsigned long long first = 0; first < total; first += cap, b++) {
 		unsigned n = (unsigned)(total - first < cap ? total - first : cap);

@@ -557,8 +557,10 @@ __device__ __forceinline__ void back64(const __bf16 *__restrict__ WT, const bf16
 }
 
 // Forward + backward over n = min(chunk, records) samples: gradients of the
-// mean of -(target / mean target) log p(dir) (uniformly drawn records) or of
-// the mean of -log p(dir) (records drawn in proportion to target) into grad.
+// mean of -(target / mean target) log q(dir) (uniformly drawn records) or of
+// the mean of -log q(dir) (records drawn in proportion to target) into grad;
+// q is the vMF mixture p, or with a.train_bsdf = b > 0 the density
+// (1 - b) p + b cos/pi that one-sample MIS with the BSDF samples.
 __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__restrict__ grid,
                                                      const __bf16 *__restrict__ wp,
                                                      const __bf16 *__restrict__ wt,
@@ -711,6 +713,15 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
     pd += __shfl_xor(pd, 16);
     pd += __shfl_xor(pd, 32);
     pd = fmaxf(pd, 1e-20f);
+    // loss -w log q with q = (1 - b) pd + b cos/pi: its gradient is that of
+    // -w log pd scaled by (1 - b) pd / q
+    float wl = wgt, qd = pd;
+    if (a.train_bsdf > 0.0f) {
+      const float cb = fmaxf(r.nrm[0] * r.dir[0] + r.nrm[1] * r.dir[1] + r.nrm[2] * r.dir[2],
+                             0.0f) * 0.318309886f;
+      qd = fmaxf((1.0f - a.train_bsdf) * pd + a.train_bsdf * cb, 1e-20f);
+      wl = wgt * ((1.0f - a.train_bsdf) * pd / qd);
+    }
     float go[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) go[k] = 0.0f;
@@ -718,19 +729,19 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
 #pragma unroll
       for (int lb = 0; lb < 2; lb++) {
         const float rsp = pi[lb] * vv[lb] / pd;  // responsibility
-        go[5 * lb] = -wgt * (rsp - pi[lb]);
+        go[5 * lb] = -wl * (rsp - pi[lb]);
         if (dirok[lb]) {
-          const float sc = -wgt * rsp * kap[lb] * il[lb];
+          const float sc = -wl * rsp * kap[lb] * il[lb];
 #pragma unroll
           for (int k = 0; k < 3; k++) go[5 * lb + 1 + k] = sc * (r.dir[k] - cth[lb] * mu[lb][k]);
         }
         const float lk = o[5 * lb + 4];
         if (lk > GD_LOGK_MIN && lk < GD_LOGK_MAX) {
           const float dl = 1.0f / kap[lb] - 2.0f / expm1f(2.0f * kap[lb]) + (cth[lb] - 1.0f);
-          go[5 * lb + 4] = -wgt * rsp * kap[lb] * dl;
+          go[5 * lb + 4] = -wl * rsp * kap[lb] * dl;
         }
       }
-      if (g == 0) lacc -= wgt * logf(pd);
+      if (g == 0) lacc -= wgt * logf(qd);
     }
     // ---- output layer: dW4, db4, dX3
 #pragma unroll

@@ -30,7 +30,7 @@ struct PtArgs {
 	uint32_t  train_cap;      /* training records per batch */
 	uint32_t  guide_depth;    /* vertices at depth < guide_depth are guided */
 	uint32_t  guide_share;    /* depth 0: one proposal per this many paths */
-	uint32_t  guide_target;   /* training target: 0 path radiance, 1 next-vertex direct */
+	float     train_bsdf;     /* training density: (1 - b) vMF + b cos/pi (0: vMF alone) */
 };
 
 /* Path state, 64 bytes. */
@@ -76,6 +76,11 @@ struct PtLight {
 /* training target: 0 = all radiance the path gathers after the vertex,
    1 = the next vertex's direct light */
 #define GD_DEFAULT_TARGET 0
+/* default training objective: 0 fits the vMF mixture alone, 1 the MIS density */
+#define GD_DEFAULT_OBJECTIVE 0
+/* warm-up: Adam steps per frame for the first frames after enable */
+#define GD_DEFAULT_WARM_FRAMES 16u
+#define GD_DEFAULT_WARM_STEPS  64u
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 

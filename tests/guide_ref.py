@@ -146,16 +146,27 @@ def pdf(o, d):
     return (pi * vmf(k, c)).sum(axis=1)
 
 
-def loss(p, rec, gmin, ginv):
-    """Mean over records of -(target / mean target) log p(dir)."""
+def cos_density(rec):
+    """Cosine BSDF sampling density max(n.d, 0) / pi of each record's direction."""
+    c = (rec["nrm"].astype(np.float64) * rec["dir"].astype(np.float64)).sum(axis=1)
+    return np.maximum(c, 0.0) / np.pi
+
+
+def loss(p, rec, gmin, ginv, bsdf=0.0):
+    """Mean over records of -(target / mean target) log q(dir), where
+    q = (1 - bsdf) p + bsdf cos/pi is the density the renderer samples
+    (bsdf = 0: the vMF mixture p alone)."""
     x = encode(p, rec["pos"], rec["nrm"], gmin, ginv)
     o, _ = forward(p, x)
     w = rec["target"].astype(np.float64)
     w = w / w.mean() / len(w)
-    return float(-(w * np.log(np.maximum(pdf(o, rec["dir"].astype(np.float64)), 1e-20))).sum())
+    q = np.maximum(pdf(o, rec["dir"].astype(np.float64)), 1e-20)
+    if bsdf > 0.0:
+        q = (1.0 - bsdf) * q + bsdf * cos_density(rec)
+    return float(-(w * np.log(q)).sum())
 
 
-def grad(p, rec, gmin, ginv):
+def grad(p, rec, gmin, ginv, bsdf=0.0):
     """Analytic gradient of loss() with respect to all parameters."""
     pos, nrm = rec["pos"], rec["nrm"]
     d = rec["dir"].astype(np.float64)
@@ -167,6 +178,9 @@ def grad(p, rec, gmin, ginv):
     c = np.einsum("nkj,nj->nk", mu, d)
     v = vmf(k, c)
     pp = np.maximum((pi * v).sum(axis=1), 1e-20)
+    if bsdf > 0.0:
+        # d(-log q) = (1 - bsdf) p / q * d(-log p)
+        w = w * (1.0 - bsdf) * pp / ((1.0 - bsdf) * pp + bsdf * cos_density(rec))
     r = pi * v / pp[:, None]
     go = np.zeros_like(o)
     go[:, 0:5 * LOBES:5] = -w[:, None] * (r - pi)

@@ -3,6 +3,7 @@ guide_ref.py): vMF mixture density normalisation, the sampling routine's
 consistency with the density, the one-sample MIS weight, and the analytic
 loss gradient against finite differences."""
 import numpy as np
+import pytest
 
 import guide_ref as G
 
@@ -78,7 +79,10 @@ def test_one_sample_mis_weight_is_unbiased():
     assert abs(w.mean() - 1.0) < 0.01, w.mean()
 
 
-def test_loss_gradient_matches_finite_differences():
+@pytest.mark.parametrize("bsdf", [0.0, 0.5])
+def test_loss_gradient_matches_finite_differences(bsdf):
+    """bsdf = 0: KL to the vMF mixture; 0.5: KL to the one-sample MIS
+    density the renderer samples."""
     rng = np.random.default_rng(6)
     p = np.zeros(G.NPARAM)
     p[:G.N_GRID] = rng.uniform(-0.3, 0.3, G.N_GRID)
@@ -94,7 +98,7 @@ def test_loss_gradient_matches_finite_differences():
     rec["dir"] = uniform_sphere(n, rng)
     rec["target"] = rng.uniform(0.1, 2.0, n)
     gmin, ginv = np.zeros(3, np.float32), np.ones(3, np.float32) * np.float32(0.9)
-    g = G.grad(p, rec, gmin, ginv)
+    g = G.grad(p, rec, gmin, ginv, bsdf)
     # check MLP weights, biases and grid entries that the records touch
     q = G.normalise(rec["pos"], gmin, ginv)
     idx, _ = G.corners(q, 5)
@@ -105,5 +109,5 @@ def test_loss_gradient_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (G.loss(pp, rec, gmin, ginv) - G.loss(pm, rec, gmin, ginv)) / (2 * h)
+        fd = (G.loss(pp, rec, gmin, ginv, bsdf) - G.loss(pm, rec, gmin, ginv, bsdf)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-5 + 2e-3 * abs(fd), (i, fd, g[i])

@@ -104,9 +104,13 @@ def test_mixture_kernel_matches_cpu(guided_scene):
     assert np.median(kerr) < 1e-2 and kerr.max() < 0.2, (np.median(kerr), kerr.max())
 
 
-def test_gradient_matches_cpu(guided_scene):
+@pytest.mark.parametrize("objective", [0, 1])
+def test_gradient_matches_cpu(guided_scene, objective):
+    """objective 0: KL to the vMF mixture; 1: KL to the MIS density
+    0.5 vMF + 0.5 cos/pi (the fixture's mix is 0.5)."""
     s = guided_scene
     L = lib()
+    check(L.rend_hip_guide_objective(s.rd, objective))
     p = random_params(3)
     check(L.rend_hip_guide_params(s.rd, ptr(p), 1))
     pos, nrm = points(s, 512, 4)
@@ -118,10 +122,14 @@ def test_gradient_matches_cpu(guided_scene):
     rec["target"] = rng.uniform(0.1, 2.0, len(pos))
     g = np.zeros(G.NPARAM, np.float32)
     loss = C.c_float(0)
-    say("guide gradient: 512 records")
-    check(L.rend_hip_guide_grad_host(s.rd, ptr(rec), len(rec), ptr(g), C.byref(loss)))
+    say(f"guide gradient: 512 records, objective {objective}")
+    try:
+        check(L.rend_hip_guide_grad_host(s.rd, ptr(rec), len(rec), ptr(g), C.byref(loss)))
+    finally:
+        L.rend_hip_guide_objective(s.rd, 0)
     gmin, ginv = G.scene_box(s.aabbs())
-    want = G.grad(p.astype(np.float64), rec, gmin, ginv)
+    bsdf = 0.5 if objective else 0.0
+    want = G.grad(p.astype(np.float64), rec, gmin, ginv, bsdf)
     for name, a, b in (("W1", G.OFS_W1, G.OFS_B1), ("W2", G.OFS_W2, G.OFS_B2),
                        ("W3", G.OFS_W3, G.OFS_B3), ("W4", G.OFS_W4, G.OFS_B4),
                        ("b4", G.OFS_B4, G.NPARAM), ("grid", 0, G.N_GRID)):
@@ -130,7 +138,7 @@ def test_gradient_matches_cpu(guided_scene):
         ratio = np.linalg.norm(u) / (np.linalg.norm(v) + 1e-30)
         say(f"  {name}: cosine {cos:.4f} norm ratio {ratio:.3f}")
         assert cos > 0.97 and 0.9 < ratio < 1.1, name
-    assert abs(loss.value - G.loss(p.astype(np.float64), rec, gmin, ginv)) < 0.05 * abs(
+    assert abs(loss.value - G.loss(p.astype(np.float64), rec, gmin, ginv, bsdf)) < 0.05 * abs(
         loss.value) + 1e-3
 
 
