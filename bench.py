@@ -284,7 +284,8 @@ def bench_guide(a):
         if guided:
             # a fresh network; the first frames train harder
             check(L.rend_hip_guide_enable(scene.rd, *cfg, 7), "guide")
-            check(L.rend_hip_guide_training(scene.rd, a.guide_chunk, cfg[3], cfg[3], 1), "schedule")
+            check(L.rend_hip_guide_training(scene.rd, a.guide_chunk, cfg[3], a.guide_lr_mlp, 1),
+                  "schedule")
             check(L.rend_hip_guide_warmup(scene.rd, a.guide_warm_frames, a.guide_warm_steps),
                   "warm-up")
         else:
@@ -317,7 +318,7 @@ def bench_guide(a):
     tp, plain, cp = run(a.guide_frames, False, ref=ref, check_every=every, tag="plain")
     err_p = rel_mse(plain, ref)
     log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.4f}")
-    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 4), stop_at=err_p,
+    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 8), stop_at=err_p,
                          tmax=3.0 * tp, tag="guided")
     err_g = rel_mse(guided, ref)
     info = (C.c_float * 4)()
@@ -340,7 +341,7 @@ def bench_guide(a):
           "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
                     "warm_frames": a.guide_warm_frames, "warm_steps_per_frame": a.guide_warm_steps,
                     "records_per_step": a.guide_chunk,
-                    "lr": cfg[3]}},
+                    "lr_grid": cfg[3], "lr_mlp": a.guide_lr_mlp}},
          higher=False)
     scene.close()
     return 0
@@ -370,7 +371,8 @@ def main():
     ap.add_argument("--guide-stride", type=int, default=32,
                     help="record a training sample at every n-th path")
     ap.add_argument("--guide-steps", type=int, default=8)
-    ap.add_argument("--guide-lr", type=float, default=1e-2)
+    ap.add_argument("--guide-lr", type=float, default=1e-2, help="hash-grid learning rate")
+    ap.add_argument("--guide-lr-mlp", type=float, default=3e-3, help="MLP learning rate")
     ap.add_argument("--guide-warm-frames", type=int, default=16)
     ap.add_argument("--guide-chunk", type=int, default=4096,
                     help="training records per Adam step")

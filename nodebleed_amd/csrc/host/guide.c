@@ -192,7 +192,7 @@ int rend_hip_guide_enable(struct rdata *rd, float mix, unsigned int train_stride
 	g->stride = train_stride ? train_stride : 4;
 	g->steps = steps;
 	g->lr_grid = lr > 0.0f ? lr : 1e-2f;
-	g->lr_mlp = g->lr_grid;
+	g->lr_mlp = GD_DEFAULT_MLP_LR_SCALE * g->lr_grid;
 	g->chunk = GD_DEFAULT_CHUNK;
 	g->importance = GD_DEFAULT_IMPORTANCE;
 	g->depth = GD_DEFAULT_DEPTH;

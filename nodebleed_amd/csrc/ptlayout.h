@@ -81,6 +81,8 @@ struct PtLight {
 /* warm-up: Adam steps per frame for the first frames after enable */
 #define GD_DEFAULT_WARM_FRAMES 16u
 #define GD_DEFAULT_WARM_STEPS  64u
+/* the MLP's learning rate relative to the hash grid's */
+#define GD_DEFAULT_MLP_LR_SCALE 0.3f
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 
