@@ -372,7 +372,7 @@ def main():
                     help="record a training sample at every n-th path")
     ap.add_argument("--guide-steps", type=int, default=8)
     ap.add_argument("--guide-lr", type=float, default=1e-2, help="hash-grid learning rate")
-    ap.add_argument("--guide-lr-mlp", type=float, default=3e-3, help="MLP learning rate")
+    ap.add_argument("--guide-lr-mlp", type=float, default=1e-3, help="MLP learning rate")
     ap.add_argument("--guide-warm-frames", type=int, default=16)
     ap.add_argument("--guide-chunk", type=int, default=4096,
                     help="training records per Adam step")

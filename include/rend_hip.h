@@ -144,7 +144,7 @@ int          rend_hip_pt_graphs(int on);
  * trained online from the vertices of every train_stride-th path: `steps`
  * Adam steps of 4K records per frame, spread over its wavefront batches,
  * records drawn in proportion to their radiance targets, learning rate lr
- * for the hash grid and 0.3 lr for the MLP.  The proposal is used from the
+ * for the hash grid and 0.1 lr for the MLP.  The proposal is used from the
  * first trained batch on. */
 int          rend_hip_guide_enable(struct rdata *rd, float mix,
                                    unsigned int train_stride, unsigned int steps,

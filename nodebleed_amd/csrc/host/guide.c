@@ -38,6 +38,7 @@ int rk_guide_train(const struct PtArgs *a, const void *grid16, const void *w16,
 int rk_guide_adam(float *master, float *grad, float *m1, float *m2,
                   void *grid16, void *w16, unsigned step, float lr_grid,
                   float lr_mlp, void *stream);
+int rk_guide_grid_mirror(const float *master, void *grid16, void *stream);
 int rk_guide_forward(const struct PtArgs *a, const void *grid16,
                      const void *w16, const float *master, const float *pn,
                      unsigned n, float *out, void *stream);
@@ -107,7 +108,8 @@ static int upload_params(struct rl_guide *g, const float *p, hipStream_t st)
 	if (e)
 		return rl_hipfail(e, "guide parameters");
 	/* zero moments, then one Adam call with lr 0 and zero gradient writes
-	 * the mirrors */
+	 * the MLP mirrors (it leaves untouched grid entries alone) and a copy
+	 * kernel the grid's */
 	if (hipMemsetAsync(g->m1, 0, GD_NPARAM * sizeof(float), st) ||
 	    hipMemsetAsync(g->m2, 0, GD_NPARAM * sizeof(float), st) ||
 	    hipMemsetAsync(g->grad, 0, GD_NPARAM * sizeof(float), st) ||
@@ -115,6 +117,8 @@ static int upload_params(struct rl_guide *g, const float *p, hipStream_t st)
 		return rl_fail(REND_ENODEV, "guide state clear");
 	int r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16, 1,
 	  0.0f, 0.0f, st);
+	if (!r)
+		r = rk_guide_grid_mirror(g->master, g->grid16, st);
 	if (r)
 		return rl_hipfail(r, "guide mirrors");
 	g->step = 0;

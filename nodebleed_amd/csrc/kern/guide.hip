@@ -1008,7 +1008,10 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
 }
 
 // Adam on the fp32 master parameters; refreshes the fp16/bf16 mirrors and
-// clears the gradient.
+// clears the gradient.  Hash-grid entries a step did not touch (gradient
+// exactly zero) keep their parameters and moments: their master copy and
+// fp16 mirror are neither read nor written, so a step costs one read of the
+// grid gradient plus the touched entries.
 __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ grad,
                              float *__restrict__ m1, float *__restrict__ m2,
                              __half *__restrict__ grid16, __bf16 *__restrict__ w16,
@@ -1018,18 +1021,17 @@ __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ gra
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < GD_NPARAM;
        i += gridDim.x * blockDim.x) {
     const float g = grad[i];
-    float p = master[i];
     if (g != 0.0f || i >= GD_N_GRID) {
       const float a = b1 * m1[i] + (1.0f - b1) * g;
       const float b = b2 * m2[i] + (1.0f - b2) * g * g;
       m1[i] = a;
       m2[i] = b;
       const float lr = i < GD_N_GRID ? lr_grid : lr_mlp;
-      p -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+      const float p = master[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
       master[i] = p;
       grad[i] = 0.0f;
+      if (i < GD_N_GRID) grid16[i] = __float2half(p);
     }
-    if (i < GD_N_GRID) grid16[i] = __float2half(p);
   }
   // bf16 mirror of the weights (not biases), packed W1 W2 W3 W4
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < (uint32_t)GD_NMLP;
@@ -1046,6 +1048,13 @@ __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ gra
     const int td = permT_dst(i);
     if (td >= 0) w16[2 * GD_NW16 + td] = (__bf16)master[i];
   }
+}
+
+// fp16 mirror of the whole hash grid from the master copy (after an upload).
+__global__ void k_guide_grid_mirror(const float *__restrict__ master, __half *__restrict__ grid16) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < GD_N_GRID;
+       i += gridDim.x * blockDim.x)
+    grid16[i] = __float2half(master[i]);
 }
 
 // Standalone forward for tests: positions/normals (6 floats each) -> raw outputs.
@@ -1171,6 +1180,12 @@ extern "C" int rk_guide_adam(float *master, float *grad, float *m1, float *m2, v
                              void *stream) {
   hipLaunchKernelGGL(k_guide_adam, dim3(4096), dim3(256), 0, (hipStream_t)stream, master, grad,
                      m1, m2, (__half *)grid16, (__bf16 *)w16, step, lr_grid, lr_mlp);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_guide_grid_mirror(const float *master, void *grid16, void *stream) {
+  hipLaunchKernelGGL(k_guide_grid_mirror, dim3(4096), dim3(256), 0, (hipStream_t)stream, master,
+                     (__half *)grid16);
   return (int)hipGetLastError();
 }
 

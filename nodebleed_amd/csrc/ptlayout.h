@@ -82,7 +82,7 @@ struct PtLight {
 #define GD_DEFAULT_WARM_FRAMES 16u
 #define GD_DEFAULT_WARM_STEPS  64u
 /* the MLP's learning rate relative to the hash grid's */
-#define GD_DEFAULT_MLP_LR_SCALE 0.3f
+#define GD_DEFAULT_MLP_LR_SCALE 0.1f
 #define GD_NMIN    16.0f
 #define GD_NMAX    1024.0f
 
