@@ -43,6 +43,17 @@ const char  *rend_hip_rcp_cpu(void);
 /* Select the reciprocal mode for later traces (default HOST_TABLE). */
 int          rend_hip_set_rcp(int mode);
 
+/* Where rend_prepdynamic builds the TLAS (rend.c:3073-3090).  HOST: the
+ * reference's SAH-binned build_bvh8, byte for byte, into rd->bnodes, then
+ * repacked and uploaded.  GPU: a device LBVH build (Morton order, radix tree,
+ * collapsed to 8-wide nodes) from rd->aabbs at the next upload, at most 4096
+ * instances; rd->bnodes' TLAS region is not written.  The trees differ, so
+ * hits are the same (up to which of two instances at exactly equal distance
+ * is reported) and visit counts are not. */
+#define REND_TLAS_HOST  0
+#define REND_TLAS_GPU   1
+int          rend_hip_set_tlas_build(int mode);
+
 /* Re-upload the scene of rd (normally done by rend_prepstatic and
  * rend_prepdynamic). */
 int          rend_hip_upload(struct rdata *rd);

@@ -34,6 +34,8 @@ REND_MODE_NORMALS = 0
 REND_MODE_PATHTRACE = 1
 REND_RCP_HOST_TABLE = 0
 REND_RCP_IEEE = 1
+REND_TLAS_HOST = 0
+REND_TLAS_GPU = 1
 INST_ID_BITS = 12
 NODE_LEAF = 0x80000000
 
@@ -71,6 +73,7 @@ _SIGS = {
     "rend_hip_rcp_cpu": (C.c_char_p, []),
     "rend_hip_devinfo": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_uint]),
     "rend_hip_set_rcp": (C.c_int, [C.c_int]),
+    "rend_hip_set_tlas_build": (C.c_int, [C.c_int]),
     "rend_hip_set_tuning": (C.c_int, [C.c_uint, C.c_uint]),
     "rend_hip_set_trace_variant": (C.c_int, [C.c_int]),
     "rend_hip_set_iter_limit": (C.c_int, [C.c_uint]),

@@ -22,6 +22,7 @@ struct rl_global {
 	unsigned            max_iter;    /* traversal step limit per ray */
 	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
 	int                 variant;     /* traversal occupancy variant */
+	int                 tlas_gpu;    /* rend_prepdynamic builds the TLAS on the device */
 	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS, then RK_DIAG_WORDS */
 	unsigned           *diag;        /* error block shared by all traversal launches */
 	unsigned            ctl_next;
@@ -80,6 +81,11 @@ struct rl_scene {
 	int                  acc_dirty;
 	int                  static_ok, dynamic_ok;
 	int                  static_dirty, dynamic_dirty, tlas_built;
+	int                  tlas_dev;   /* the pending TLAS is built on the device */
+	void                *aabbs_dev;  /* instance world AABBs for the device build */
+	float               *tlas_work;  /* device build scratch */
+	unsigned            *tlas_info;  /* device build result: nodes, error */
+	size_t               aabbs_cap;
 	unsigned             hazards;
 	struct rl_frame     *frame;
 	struct rl_pt        *pt;

@@ -120,6 +120,14 @@ int rend_hip_set_rcp(int mode)
 	return 0;
 }
 
+int rend_hip_set_tlas_build(int mode)
+{
+	if (mode != REND_TLAS_HOST && mode != REND_TLAS_GPU)
+		return rl_fail(REND_EINVAL, "TLAS build mode %d", mode);
+	G.tlas_gpu = mode == REND_TLAS_GPU;
+	return 0;
+}
+
 int rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill)
 {
 	if (refill < 1 || refill > 64)
@@ -215,6 +223,9 @@ static void scene_del(struct rdata *rd)
 			struct rl_scene *s = g_scenes[i];
 			dfree((void **)&s->arena);
 			dfree((void **)&s->insts);
+			dfree((void **)&s->aabbs_dev);
+			dfree((void **)&s->tlas_work);
+			dfree((void **)&s->tlas_info);
 			dfree((void **)&s->nrms);
 			dfree((void **)&s->mtls);
 			dfree((void **)&s->tris);
@@ -474,10 +485,95 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 	return 0;
 }
 
+int rk_tlas_build(const void *aabbs, unsigned n, void *work, void *nodes, unsigned *info,
+                  void *stream);
+unsigned rk_tlas_work_floats(unsigned n);
+
+/* Instances and a TLAS built on the device from the instances' world AABBs
+ * (kern/tlas.hip) into the arena after the BLASes. */
+static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
+{
+	unsigned n = rd->instcnt;
+	if (n > 4096)
+		return rl_fail(REND_EINVAL, "%u instances (device TLAS build: at most 4096)", n);
+	size_t sz = (size_t)(n ? n : 1) * sizeof(struct bnode8);
+	size_t at = (s->blas_bytes + 255) & ~(size_t)255;
+	int r = arena_reserve(s, at + sz, s->blas_bytes);
+	if (r)
+		return r;
+	struct dinst *di = malloc(((size_t)n + 1) * sizeof(*di));
+	if (!di)
+		return rl_fail(REND_ENOMEM, "instance staging");
+	for (unsigned i = 0; i < n; i++) {
+		const struct rinst *ri = &rd->insts[i];
+		int b = blas_find(s, ri->triofs);
+		if (b < 0) {
+			free(di);
+			return rl_fail(REND_ESTATE,
+			  "instance %u references a BLAS not built by rend_prepstatic", i);
+		}
+		memcpy(di[i].inv, ri->globinv, sizeof(di[i].inv));
+		di[i].blas = s->blas_dev[b];
+		di[i].triofs = ri->triofs;
+		di[i].flags = ri->flags;
+	}
+	hipError_t e = hipSuccess;
+	size_t ib = ((size_t)n + 1) * sizeof(*di);
+	size_t ab = ((size_t)n + 1) * sizeof(struct aabb);
+	if (ib > s->insts_cap) {
+		dfree((void **)&s->insts);
+		s->insts_cap = 0;
+		if ((e = hipMalloc((void **)&s->insts, ib)) == hipSuccess)
+			s->insts_cap = ib;
+	}
+	if (e == hipSuccess && ab > s->aabbs_cap) {
+		dfree(&s->aabbs_dev);
+		dfree((void **)&s->tlas_work);
+		s->aabbs_cap = 0;
+		if ((e = hipMalloc(&s->aabbs_dev, ab)) == hipSuccess &&
+		    (e = hipMalloc((void **)&s->tlas_work,
+		      (size_t)rk_tlas_work_floats(n + 1) * sizeof(float))) == hipSuccess)
+			s->aabbs_cap = ab;
+	}
+	if (e == hipSuccess && !s->tlas_info)
+		e = hipMalloc((void **)&s->tlas_info, 4 * sizeof(unsigned));
+	if (e == hipSuccess && n)
+		e = hipMemcpyAsync(s->insts, di, (size_t)n * sizeof(*di), hipMemcpyHostToDevice,
+		  G.stream);
+	if (e == hipSuccess && n)
+		e = hipMemcpyAsync(s->aabbs_dev, rd->aabbs, (size_t)n * sizeof(struct aabb),
+		  hipMemcpyHostToDevice, G.stream);
+	if (e != hipSuccess) {
+		free(di);
+		return rl_hipfail(e, "device TLAS inputs");
+	}
+	r = rk_tlas_build(s->aabbs_dev, n, s->tlas_work, s->arena + at, s->tlas_info, G.stream);
+	unsigned info[2] = {0, 0};
+	if (!r)
+		r = (int)hipMemcpyAsync(info, s->tlas_info, sizeof(info), hipMemcpyDeviceToHost,
+		  G.stream);
+	if (!r)
+		r = (int)hipStreamSynchronize(G.stream);
+	free(di);
+	if (r)
+		return rl_hipfail(r, "device TLAS build");
+	if (info[1] || !info[0] || (size_t)info[0] * sizeof(struct bnode8) > sz)
+		return rl_fail(REND_EINVAL, "device TLAS build failed (nodes %u, code %u)",
+		  info[0], info[1]);
+	s->tlas = at;
+	s->tlas_bytes = (size_t)info[0] * sizeof(struct bnode8);
+	s->arena_used = at + s->tlas_bytes;
+	s->ninsts = n;
+	s->dynamic_ok = 1;
+	return 0;
+}
+
 static int upload_dynamic(struct rdata *rd, struct rl_scene *s)
 {
 	if (!s->static_ok)
 		return rl_fail(REND_ESTATE, "rend_prepdynamic before rend_prepstatic");
+	if (s->tlas_dev)
+		return upload_dynamic_dev(rd, s);
 	const struct bnode8 *root = &rd->bnodes[rd->tlasofs];
 	size_t limit = 2 * (size_t)(s->maxinsts ? s->maxinsts : 1) *
 	  sizeof(struct bnode8);
@@ -592,6 +688,19 @@ void rend_prepstatic(struct rdata *rd)
 
 void rend_prepdynamic(struct rdata *rd)
 {
+	if (G.tlas_gpu) {
+		/* built on the device at the next upload; the host arena's TLAS
+		 * region is left as it was */
+		struct rl_scene *s = scene_add(rd);
+		if (!s) {
+			rl_fail(REND_ENOMEM, "scene registry full");
+			return;
+		}
+		s->tlas_dev = 1;
+		s->tlas_built = 1;
+		s->dynamic_dirty = 1;
+		return;
+	}
 	unsigned n = rd->instcnt;
 	unsigned *imap = malloc(((size_t)n + 1) * sizeof(*imap));
 	if (!imap) {
@@ -615,6 +724,7 @@ void rend_prepdynamic(struct rdata *rd)
 		return;
 	}
 	s->hazards |= st.hazards;
+	s->tlas_dev = 0;
 	s->tlas_built = 1;
 	s->dynamic_dirty = 1;
 }
