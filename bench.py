@@ -177,6 +177,64 @@ def bench_pt(a):
     return 0
 
 
+def bench_tlas(a):
+    """Per-frame TLAS rebuild of the instanced scene (4,096 instances over 16
+    meshes): rend_prepdynamic + upload with the reference's host SAH build
+    (build_bvh8, repack, upload) against the device LBVH build, and the
+    closest-hit throughput of 1920x1080 diffuse rays through either tree.
+    Step = one rebuild."""
+    from nodebleed_amd import REND_TLAS_GPU, REND_TLAS_HOST
+    L = lib()
+    scene = Scene(config=3, seed=a.seed, detail=1.0, width=W, height=H, spp=1)
+    ninst = scene.view().instcnt
+    log(f"scene: {scene.ntris} tris, {ninst} instances")
+    n = W * H
+    bufs = [L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize),
+            L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize)]
+    d_prim, d_phit, d_diff, d_dhit = bufs
+    stream = L.rend_hip_stream()
+    check(L.rend_hip_gen_camrays(scene.rd, d_prim, W, H, 1, a.seed, stream), "camrays")
+    check(L.rend_hip_trace(scene.rd, d_prim, d_phit, n, stream, None), "primary trace")
+    check(L.rend_hip_gen_diffuse(scene.rd, d_prim, d_phit, d_diff, n, a.seed, stream),
+          "diffuse rays")
+    check(L.rend_hip_sync(), "sync")
+    res = {}
+    try:
+        for name, mode in (("host", REND_TLAS_HOST), ("device", REND_TLAS_GPU)):
+            check(L.rend_hip_set_tlas_build(mode), "tlas mode")
+            for _ in range(max(1, a.warmup)):
+                L.rend_prepdynamic(C.c_void_p(scene.rd))
+                check(L.rend_hip_sync_scene(scene.rd), "rebuild")
+            t = time.perf_counter()
+            for _ in range(a.steps):
+                L.rend_prepdynamic(C.c_void_p(scene.rd))
+                check(L.rend_hip_sync_scene(scene.rd), "rebuild")
+            ms = (time.perf_counter() - t) * 1e3 / a.steps
+            st = TraceStats()
+            check(L.rend_hip_trace(scene.rd, d_diff, d_dhit, n, stream, C.byref(st)), "count")
+            timed_launches(L, scene, d_diff, d_dhit, n, 2)
+            tr = timed_launches(L, scene, d_diff, d_dhit, n, a.steps) / a.steps
+            res[name] = {"rebuild_ms": round(ms, 4), "diffuse_trace_ms": round(tr, 4),
+                         "diffuse_mrays_s": round(n / tr / 1e3, 1),
+                         "visits_per_ray": {"nodes": round(st.nodes / n, 3),
+                                            "leaves": round(st.leaves / n, 3),
+                                            "instances": round(st.insts / n, 3)}}
+            log(f"{name} TLAS: rebuild {ms:.3f} ms, diffuse trace {tr:.3f} ms")
+    finally:
+        L.rend_hip_set_tlas_build(REND_TLAS_HOST)
+        for p in bufs:
+            L.rend_hip_free(p)
+        scene.close()
+    dev = res["device"]["rebuild_ms"]
+    emit("TLAS rebuild ms (rend_prepdynamic + upload), device build", dev, "ms",
+         a.steps, a.warmup, dev,
+         {"workload": f"TLAS over {ninst} instances of 16 meshes (50k-500k tris), rebuilt "
+                      "per frame; diffuse rays 1920x1080 traced through each tree",
+          "host": res["host"], "device": res["device"],
+          "speedup": round(res["host"]["rebuild_ms"] / dev, 1)}, higher=False)
+    return 0
+
+
 def uneven_views(seed=1, n=32):
     """n cameras over the terrain: resolutions from {320x240 .. 3840x2160},
     spp uniform in 4..64, eyes on a ring above the terrain."""
@@ -360,10 +418,11 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after timing, run one plain copy kernel of MIB MiB "
                          "(reference for hardware byte counters)")
-    ap.add_argument("--mode", default="trace", choices=("trace", "pt", "views", "guide"),
+    ap.add_argument("--mode", default="trace", choices=("trace", "pt", "views", "guide", "tlas"),
                     help="trace: closest-hit microbench (the headline line); pt: "
                          "instanced path tracing; views: 32 uneven cameras in one batched "
-                         "launch; guide: equal-time guided vs plain path tracing")
+                         "launch; guide: equal-time guided vs plain path tracing; tlas: "
+                         "per-frame TLAS rebuild, host SAH vs device LBVH")
     ap.add_argument("--guide-frames", type=int, default=256)
     ap.add_argument("--guide-ref-frames", type=int, default=4096)
     ap.add_argument("--guide-spf", type=int, default=16, help="samples per pixel per frame")
@@ -384,7 +443,8 @@ def main():
               "multi-GPU sharding is a design note, not an implemented path", file=sys.stderr)
         sys.exit(2)
     if a.mode != "trace":
-        sys.exit({"pt": bench_pt, "views": bench_views, "guide": bench_guide}[a.mode](a))
+        sys.exit({"pt": bench_pt, "views": bench_views, "guide": bench_guide,
+                  "tlas": bench_tlas}[a.mode](a))
 
     L = lib()
     t0 = time.perf_counter()

@@ -57,6 +57,10 @@ int          rend_hip_set_tlas_build(int mode);
 /* Re-upload the scene of rd (normally done by rend_prepstatic and
  * rend_prepdynamic). */
 int          rend_hip_upload(struct rdata *rd);
+/* Upload (or, with REND_TLAS_GPU, build on the device) what rend_prepstatic
+ * and rend_prepdynamic changed since the last upload, and wait; the next
+ * trace or frame would do this itself. */
+int          rend_hip_sync_scene(struct rdata *rd);
 
 /* Closest hit for n rays of d_rays into d_hits (device buffers). Reference
  * semantics: intersect_tlas (rend.c:1976-2155) per ray, starting t = tmax.

@@ -827,6 +827,16 @@ static int check_ready(struct rdata *rd, struct rl_scene **out)
 	return 0;
 }
 
+int rend_hip_sync_scene(struct rdata *rd)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	hipError_t e = hipStreamSynchronize(G.stream);
+	return e ? rl_hipfail(e, "scene upload") : 0;
+}
+
 /* Reads and clears the error word of the last launch (synchronises). */
 int rl_check_errors(hipStream_t st)
 {

@@ -3,9 +3,10 @@ reference's host-built SAH TLAS (rend_prepdynamic, rend.c:3073-3090).
 
 The trees differ, the hits may not.  Bar: id, t, u and v bit-exact with the
 oracle's traversal of the host tree, except where two hits are tied to
-within rounding (at most 1e-3 of the rays, distances within 1e-4 relative):
-the traversal culls against the running closest distance, so which of two
-such hits is reported depends on visit order.  Any-hit flags are equal.  The scene is created with
+within the triangle test's reciprocal error (at most 1e-3 of the rays,
+distances within 7.4e-4 relative): the traversal culls against the running
+closest distance, so which of two such hits is reported depends on visit
+order.  Any-hit flags are equal.  The scene is created with
 the host build, so the oracle reads the reference's tree from rd->bnodes;
 the device build leaves that region alone.
 """
@@ -60,13 +61,15 @@ def assert_same_hits(s, rays, tag, pairs=False):
     for i in np.flatnonzero(other)[:8]:
         say(f"    {tag} ray {i}: got id {got['id'][i]:#x} t {got['t'][i]!r}, "
             f"oracle id {want['id'][i]:#x} t {want['t'][i]!r}")
-    # Another hit may be reported only where two hits are (nearly) tied:
-    # boxes are culled against the running closest distance with rounded
-    # slab distances, so of two hits within rounding of each other the one
-    # visited first can hide the other, and the trees visit in other orders.
+    # Another hit may be reported only where two hits are (nearly) tied.
+    # A triangle's t comes from the 12-bit RCPSS reciprocal (relative error
+    # up to 1.5 * 2^-12 = 3.7e-4) while boxes are culled against the running
+    # closest t with accurate slab distances: of two hits within that error
+    # of each other, the one visited first can cull the other's box, and the
+    # trees visit in other orders.  Two such t differ by at most 7.4e-4.
     g, w = got[other], want[other]
     assert ((g["id"] != MISS) & (w["id"] != MISS)).all(), f"{tag}: hit against miss"
-    near = np.abs(g["t"].astype(np.float64) - w["t"]) <= 1e-4 * np.maximum(g["t"], w["t"])
+    near = np.abs(g["t"].astype(np.float64) - w["t"]) <= 7.4e-4 * np.maximum(g["t"], w["t"])
     assert near.all(), f"{tag}: {int((~near).sum())} hits differ by more than a near tie"
     if pairs:
         assert ((g["id"] >> 12) == (w["id"] >> 12)).all() and (
