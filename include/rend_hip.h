@@ -61,6 +61,11 @@ int          rend_hip_upload(struct rdata *rd);
  * and rend_prepdynamic changed since the last upload, and wait; the next
  * trace or frame would do this itself. */
 int          rend_hip_sync_scene(struct rdata *rd);
+/* The device TLAS as the traversal reads it (after any pending upload or
+ * device build): nodes receives its node count, dst (cap bytes, may be NULL)
+ * the nodes in the arena layout (dlayout.h: struct dnode, 256 bytes). */
+int          rend_hip_tlas_download(struct rdata *rd, void *dst, unsigned long long cap,
+                                    unsigned int *nodes);
 
 /* Closest hit for n rays of d_rays into d_hits (device buffers). Reference
  * semantics: intersect_tlas (rend.c:1976-2155) per ray, starting t = tmax.

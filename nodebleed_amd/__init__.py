@@ -75,6 +75,8 @@ _SIGS = {
     "rend_hip_set_rcp": (C.c_int, [C.c_int]),
     "rend_hip_set_tlas_build": (C.c_int, [C.c_int]),
     "rend_hip_sync_scene": (C.c_int, [C.c_void_p]),
+    "rend_hip_tlas_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong,
+                                         C.POINTER(C.c_uint)]),
     "rend_hip_set_tuning": (C.c_int, [C.c_uint, C.c_uint]),
     "rend_hip_set_trace_variant": (C.c_int, [C.c_int]),
     "rend_hip_set_iter_limit": (C.c_int, [C.c_uint]),

@@ -827,6 +827,26 @@ static int check_ready(struct rdata *rd, struct rl_scene **out)
 	return 0;
 }
 
+int rend_hip_tlas_download(struct rdata *rd, void *dst, unsigned long long cap,
+                           unsigned int *nodes)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	if (nodes)
+		*nodes = (unsigned)(s->tlas_bytes / sizeof(struct bnode8));
+	if (!dst)
+		return 0;
+	if (cap < s->tlas_bytes)
+		return rl_fail(REND_EINVAL, "TLAS is %zu bytes, buffer %llu", s->tlas_bytes, cap);
+	hipError_t e = hipMemcpyAsync(dst, s->arena + s->tlas, s->tlas_bytes,
+	  hipMemcpyDeviceToHost, G.stream);
+	if (!e)
+		e = hipStreamSynchronize(G.stream);
+	return e ? rl_hipfail(e, "TLAS download") : 0;
+}
+
 int rend_hip_sync_scene(struct rdata *rd)
 {
 	struct rl_scene *s;

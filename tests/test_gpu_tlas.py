@@ -159,6 +159,83 @@ def test_coincident_instances_tie():
         s.close()
 
 
+DNODE = np.dtype([("minx", "<f4", 8), ("maxx", "<f4", 8), ("miny", "<f4", 8), ("maxy", "<f4", 8),
+                  ("minz", "<f4", 8), ("maxz", "<f4", 8), ("child", "<u4", 8), ("inv", "<u4", 8)])
+FMAX = np.float32(3.4028235e38)
+
+
+def device_tree(s):
+    L = lib()
+    try:
+        rebuild(s, REND_TLAS_GPU)
+        nn = C.c_uint(0)
+        check(L.rend_hip_tlas_download(s.rd, None, 0, C.byref(nn)), "tlas size")
+        out = np.zeros(nn.value, DNODE)
+        check(L.rend_hip_tlas_download(s.rd, out.ctypes.data, out.nbytes, C.byref(nn)),
+              "tlas download")
+    finally:
+        L.rend_hip_set_tlas_build(REND_TLAS_HOST)
+    return out
+
+
+@pytest.mark.parametrize("case", ["instances", "inflated", "k9", "k300", "coincident"])
+def test_device_tree_structure(case):
+    """The downloaded device TLAS: every instance in exactly one leaf slot,
+    every node but the root referenced once at a 256-byte offset, child
+    boxes equal to the union of their subtree's instance boxes, per-octant
+    ranks a permutation of 0..7, empty slots the reference's empty box with
+    a zero word, at most n - 1 nodes."""
+    if case in ("instances", "inflated"):
+        s = Scene(config=3, detail=0.03)
+    else:
+        s = small_scene({"k9": 9, "k300": 300, "coincident": 16}[case], seed=3,
+                        duplicates=case == "coincident")
+    try:
+        ab = s.aabbs()
+        if case == "inflated":
+            c = 0.5 * (ab["min"] + ab["max"])
+            ab["min"], ab["max"] = c - 3 * (c - ab["min"]), c + 3 * (ab["max"] - c)
+        lo, hi = np.array(ab["min"]), np.array(ab["max"])
+        say(f"device tree structure: {case}, {len(lo)} instances")
+        t = device_tree(s)
+        n = len(lo)
+        assert 1 <= len(t) <= max(n - 1, 1)
+        seen, refs = np.zeros(n, int), np.zeros(len(t), int)
+
+        def walk(i, depth):
+            assert depth < 64
+            nd = t[i]
+            blo, bhi = np.full(3, FMAX), np.full(3, -FMAX)
+            for o in range(8):
+                ranks = [(int(nd["inv"][o]) >> (3 * c)) & 7 for c in range(8)]
+                assert sorted(ranks) == list(range(8)), (i, o, ranks)
+            for c in range(8):
+                w = int(nd["child"][c])
+                box_lo = np.array([nd["minx"][c], nd["miny"][c], nd["minz"][c]])
+                box_hi = np.array([nd["maxx"][c], nd["maxy"][c], nd["maxz"][c]])
+                if w == 0:
+                    assert (box_lo == FMAX).all() and (box_hi == -FMAX).all(), (i, c)
+                    continue
+                if w & 0x80000000:
+                    k = w & 0x7FFFFFFF
+                    assert k < n
+                    seen[k] += 1
+                    sub_lo, sub_hi = lo[k], hi[k]
+                else:
+                    assert w % 256 == 0 and 0 < w // 256 < len(t), (i, c, w)
+                    refs[w // 256] += 1
+                    sub_lo, sub_hi = walk(w // 256, depth + 1)
+                assert (box_lo == sub_lo).all() and (box_hi == sub_hi).all(), (i, c)
+                blo, bhi = np.minimum(blo, sub_lo), np.maximum(bhi, sub_hi)
+            return blo, bhi
+
+        walk(0, 0)
+        assert (seen == 1).all(), "every instance exactly once"
+        assert refs[0] == 0 and (refs[1:] == 1).all(), "every node referenced once"
+    finally:
+        s.close()
+
+
 def test_rebuild_after_moving_instances():
     """Instances moved between builds (new globinv and boxes, as upd_rinsts
     does each frame): the device tree follows."""
