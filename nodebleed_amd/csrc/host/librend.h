@@ -83,7 +83,6 @@ struct rl_scene {
 	int                  static_dirty, dynamic_dirty, tlas_built;
 	int                  tlas_dev;   /* the pending TLAS is built on the device */
 	void                *aabbs_dev;  /* instance world AABBs for the device build */
-	float               *tlas_work;  /* device build scratch */
 	unsigned            *tlas_info;  /* device build result: nodes, error */
 	size_t               aabbs_cap;
 	unsigned             hazards;

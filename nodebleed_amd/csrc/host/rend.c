@@ -224,7 +224,6 @@ static void scene_del(struct rdata *rd)
 			dfree((void **)&s->arena);
 			dfree((void **)&s->insts);
 			dfree((void **)&s->aabbs_dev);
-			dfree((void **)&s->tlas_work);
 			dfree((void **)&s->tlas_info);
 			dfree((void **)&s->nrms);
 			dfree((void **)&s->mtls);
@@ -485,9 +484,7 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 	return 0;
 }
 
-int rk_tlas_build(const void *aabbs, unsigned n, void *work, void *nodes, unsigned *info,
-                  void *stream);
-unsigned rk_tlas_work_floats(unsigned n);
+int rk_tlas_build(const void *aabbs, unsigned n, void *nodes, unsigned *info, void *stream);
 
 /* Instances and a TLAS built on the device from the instances' world AABBs
  * (kern/tlas.hip) into the arena after the BLASes. */
@@ -528,11 +525,8 @@ static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
 	}
 	if (e == hipSuccess && ab > s->aabbs_cap) {
 		dfree(&s->aabbs_dev);
-		dfree((void **)&s->tlas_work);
 		s->aabbs_cap = 0;
-		if ((e = hipMalloc(&s->aabbs_dev, ab)) == hipSuccess &&
-		    (e = hipMalloc((void **)&s->tlas_work,
-		      (size_t)rk_tlas_work_floats(n + 1) * sizeof(float))) == hipSuccess)
+		if ((e = hipMalloc(&s->aabbs_dev, ab)) == hipSuccess)
 			s->aabbs_cap = ab;
 	}
 	if (e == hipSuccess && !s->tlas_info)
@@ -547,7 +541,7 @@ static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
 		free(di);
 		return rl_hipfail(e, "device TLAS inputs");
 	}
-	r = rk_tlas_build(s->aabbs_dev, n, s->tlas_work, s->arena + at, s->tlas_info, G.stream);
+	r = rk_tlas_build(s->aabbs_dev, n, s->arena + at, s->tlas_info, G.stream);
 	unsigned info[2] = {0, 0};
 	if (!r)
 		r = (int)hipMemcpyAsync(info, s->tlas_info, sizeof(info), hipMemcpyDeviceToHost,

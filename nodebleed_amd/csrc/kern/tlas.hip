@@ -12,7 +12,9 @@
 //      in LDS.
 //   2. Binary radix tree over the sorted keys (Karras 2012, "Maximizing
 //      parallelism in the construction of BVHs, octrees and k-d trees").
-//   3. Internal node bounds level by level, deepest level first.
+//   3. Internal node bounds level by level, deepest level first, kept in LDS
+//      (they replace the keys, which the tree no longer needs; the whole
+//      build lives in 152 KB of LDS).
 //   4. Collapse to 8-wide nodes breadth-first: starting from a binary node's
 //      two children, the child with the largest surface area is replaced by
 //      its two children until there are eight or only leaves remain (the same
@@ -33,10 +35,13 @@ namespace {
 constexpr uint32_t kMax = 4096;       // instances (12-bit instance ids)
 constexpr uint32_t kThreads = 1024;
 constexpr uint16_t kLeaf = 0x8000u;   // child code: leaf (sorted position) flag
-constexpr uint32_t kWork = 8;         // floats per internal node in `work`
 
 struct BuildLds {
-  uint64_t key[kMax];                 // sorted Morton | instance keys
+  union {
+    uint64_t key[kMax];               // sorted Morton | instance keys (steps 1-2)
+    float nb[kMax][6];                // internal node bounds (steps 3-4)
+  };
+  uint16_t inst[kMax];                // instance at each sorted position
   uint16_t left[kMax], right[kMax];   // binary children (kLeaf | position)
   uint16_t parent[kMax];              // parent of an internal node
   uint8_t depth[kMax];                // internal node depth
@@ -64,15 +69,14 @@ struct Box {
   float lo[3], hi[3];
 };
 
-__device__ __forceinline__ Box leaf_box(const float4 *ab, const uint64_t *key, uint32_t pos) {
-  const uint32_t inst = (uint32_t)key[pos];
-  const float4 a = ab[2 * inst], b = ab[2 * inst + 1];
+__device__ __forceinline__ Box leaf_box(const float4 *ab, const uint16_t *inst, uint32_t pos) {
+  const uint32_t k = inst[pos];
+  const float4 a = ab[2 * k], b = ab[2 * k + 1];
   return {{a.x, a.y, a.z}, {b.x, b.y, b.z}};
 }
 
-__device__ __forceinline__ Box node_box(const float *work, uint32_t i) {
-  const float *w = work + (size_t)i * kWork;
-  return {{w[0], w[1], w[2]}, {w[3], w[4], w[5]}};
+__device__ __forceinline__ Box node_box(const float (*nb)[6], uint32_t i) {
+  return {{nb[i][0], nb[i][1], nb[i][2]}, {nb[i][3], nb[i][4], nb[i][5]}};
 }
 
 __device__ __forceinline__ float half_area(const Box &b) {
@@ -80,14 +84,18 @@ __device__ __forceinline__ float half_area(const Box &b) {
   return x * y + y * z + z * x;
 }
 
-__device__ __forceinline__ Box child_box(const float4 *ab, const float *work, const uint64_t *key,
-                                         uint16_t c) {
-  return (c & kLeaf) ? leaf_box(ab, key, c & ~kLeaf) : node_box(work, c);
+__device__ __forceinline__ Box child_box(const float4 *ab, const BuildLds &S, uint16_t c) {
+  return (c & kLeaf) ? leaf_box(ab, S.inst, c & ~kLeaf) : node_box(S.nb, c);
+}
+
+__device__ __forceinline__ float child_area(const BuildLds &S, uint16_t c) {
+  return (c & kLeaf) ? -1.0f : half_area(node_box(S.nb, c));
 }
 
 // One 8-wide node: bounds and words of `cnt` children, empty slots after.
+// (All loops fully unrolled so the small arrays stay in registers.)
 __device__ void write_node(dnode *out, const Box *bx, const uint32_t *word, uint32_t cnt) {
-  float dist[8][8];
+#pragma unroll
   for (uint32_t c = 0; c < 8; c++) {
     const bool e = c >= cnt;
     out->minx[c] = e ? 3.402823466e38f : bx[c].lo[0];
@@ -97,25 +105,27 @@ __device__ void write_node(dnode *out, const Box *bx, const uint32_t *word, uint
     out->minz[c] = e ? 3.402823466e38f : bx[c].lo[2];
     out->maxz[c] = e ? -3.402823466e38f : bx[c].hi[2];
     out->child[c] = e ? 0u : word[c];
-    for (uint32_t o = 0; o < 8; o++) {
-      if (e) {
-        dist[o][c] = 3.402823466e38f;
-        continue;
-      }
+  }
+#pragma unroll
+  for (uint32_t o = 0; o < 8; o++) {
+    float dist[8];
+#pragma unroll
+    for (uint32_t c = 0; c < 8; c++) {
       // near corner along the octant direction (bvh8.c ordering)
       const float x = (o & 1) ? bx[c].lo[0] : bx[c].hi[0];
       const float y = (o & 2) ? bx[c].lo[1] : bx[c].hi[1];
       const float z = (o & 4) ? bx[c].lo[2] : bx[c].hi[2];
-      dist[o][c] = ((o & 1) ? x : -x) + ((o & 2) ? y : -y) + ((o & 4) ? z : -z);
+      dist[c] = c >= cnt ? 3.402823466e38f
+                         : ((o & 1) ? x : -x) + ((o & 2) ? y : -y) + ((o & 4) ? z : -z);
     }
-  }
-  // rank = position in the stable descending order (0 farthest, 7 nearest)
-  for (uint32_t o = 0; o < 8; o++) {
+    // rank = position in the stable descending order (0 farthest, 7 nearest)
     uint32_t inv = 0;
+#pragma unroll
     for (uint32_t c = 0; c < 8; c++) {
       uint32_t r = 0;
+#pragma unroll
       for (uint32_t k = 0; k < 8; k++)
-        r += (dist[o][k] > dist[o][c] || (dist[o][k] == dist[o][c] && k < c)) ? 1u : 0u;
+        r += (dist[k] > dist[c] || (dist[k] == dist[c] && k < c)) ? 1u : 0u;
       inv |= r << (3 * c);
     }
     out->inv[o] = inv;
@@ -132,7 +142,6 @@ __device__ __forceinline__ float block_min(BuildLds &S, int slot, float v) {
 }
 
 __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ ab, uint32_t n,
-                                                     float *__restrict__ work,
                                                      dnode *__restrict__ out,
                                                      uint32_t *__restrict__ info) {
   __shared__ BuildLds S;
@@ -140,8 +149,8 @@ __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ 
   if (n < 2) {
     // root with zero or one instance leaf (build_bvh8 also makes a root node)
     if (tid == 0) {
-      Box b[1];
-      uint32_t w[1] = {DNODE_LEAF};
+      Box b[8] = {};
+      uint32_t w[8] = {DNODE_LEAF, 0, 0, 0, 0, 0, 0, 0};
       if (n) {
         const float4 a = ab[0], c = ab[1];
         b[0] = {{a.x, a.y, a.z}, {c.x, c.y, c.z}};
@@ -236,9 +245,11 @@ __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ 
     if (!(lc & kLeaf)) S.parent[lc] = (uint16_t)i;
     if (!(rc & kLeaf)) S.parent[rc] = (uint16_t)i;
   }
+  for (uint32_t p = tid; p < n; p += kThreads) S.inst[p] = (uint16_t)S.key[p];
   if (tid == 0) S.parent[0] = 0xffffu;
   __syncthreads();
-  // ---- 3. depths, then bounds from the deepest level up
+  // ---- 3. depths, then bounds from the deepest level up (the keys are dead
+  // from here on: S.nb overwrites them)
   for (uint32_t i = tid; i + 1 < n; i += kThreads) {
     uint32_t d = 0, p = i;
     while (p != 0 && p < kMax && d < kMax) {
@@ -263,12 +274,11 @@ __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ 
   for (int lv = (int)S.maxd; lv >= 0; lv--) {
     for (uint32_t i = tid; i + 1 < n; i += kThreads) {
       if (S.depth[i] != (uint32_t)lv) continue;
-      const Box a = child_box(ab, work, S.key, S.left[i]);
-      const Box b = child_box(ab, work, S.key, S.right[i]);
-      float *w = work + (size_t)i * kWork;
+      const Box a = child_box(ab, S, S.left[i]);
+      const Box b = child_box(ab, S, S.right[i]);
       for (int k = 0; k < 3; k++) {
-        w[k] = fminf(a.lo[k], b.lo[k]);
-        w[3 + k] = fmaxf(a.hi[k], b.hi[k]);
+        S.nb[i][k] = fminf(a.lo[k], b.lo[k]);
+        S.nb[i][3 + k] = fmaxf(a.hi[k], b.hi[k]);
       }
     }
     __syncthreads();
@@ -279,36 +289,48 @@ __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ 
   __syncthreads();
   for (uint32_t level = 0; qn > 0 && level < kMax; level++) {
     for (uint32_t e = tid; e < qn; e += kThreads) {
-      uint16_t ch[8];
-      float area[8];
+      // child codes and areas (leaves -1); indexed only by unrolled
+      // constants so they stay in registers
+      uint16_t ch[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      float area[8] = {-1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f, -1.0f};
       const uint16_t b = S.q[cur][e];
       ch[0] = S.left[b];
       ch[1] = S.right[b];
+      area[0] = child_area(S, ch[0]);
+      area[1] = child_area(S, ch[1]);
       uint32_t cnt = 2;
-      for (int k = 0; k < 2; k++)
-        area[k] = (ch[k] & kLeaf) ? -1.0f : half_area(node_box(work, ch[k]));
-      while (cnt < 8) {
+#pragma unroll
+      for (uint32_t it = 0; it < 6; it++) {  // at most 6 splits from 2 to 8
         int best = -1;
         float ba = -1.0f;
-        for (uint32_t k = 0; k < cnt; k++)
-          if (!(ch[k] & kLeaf) && area[k] > ba) {
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++)
+          if (k < cnt && !(ch[k] & kLeaf) && area[k] > ba) {
             ba = area[k];
             best = (int)k;
           }
         if (best < 0) break;
-        const uint16_t x = ch[best];
-        ch[best] = S.left[x];
-        ch[cnt] = S.right[x];
-        area[best] = (ch[best] & kLeaf) ? -1.0f : half_area(node_box(work, ch[best]));
-        area[cnt] = (ch[cnt] & kLeaf) ? -1.0f : half_area(node_box(work, ch[cnt]));
+        uint16_t x = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++)
+          if ((int)k == best) x = ch[k];
+        const uint16_t l = S.left[x], r = S.right[x];
+        const float al = child_area(S, l), ar = child_area(S, r);
+#pragma unroll
+        for (uint32_t k = 0; k < 8; k++) {
+          if ((int)k == best) { ch[k] = l; area[k] = al; }
+          if (k == cnt) { ch[k] = r; area[k] = ar; }
+        }
         cnt++;
       }
-      Box bx[8];
-      uint32_t word[8];
-      for (uint32_t k = 0; k < cnt; k++) {
-        bx[k] = child_box(ab, work, S.key, ch[k]);
+      Box bx[8] = {};
+      uint32_t word[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++) {
+        if (k >= cnt) continue;
+        bx[k] = child_box(ab, S, ch[k]);
         if (ch[k] & kLeaf) {
-          word[k] = DNODE_LEAF | (uint32_t)S.key[ch[k] & ~kLeaf];
+          word[k] = DNODE_LEAF | (uint32_t)S.inst[ch[k] & ~kLeaf];
         } else {
           const uint32_t slot = atomicAdd(&S.cnt, 1u);
           S.q[cur ^ 1][slot] = ch[k];
@@ -333,12 +355,10 @@ __global__ __launch_bounds__(1024) void k_tlas_build(const float4 *__restrict__ 
 
 }  // namespace
 
-extern "C" int rk_tlas_build(const void *aabbs, unsigned n, void *work, void *nodes,
-                             unsigned *info, void *stream) {
+extern "C" int rk_tlas_build(const void *aabbs, unsigned n, void *nodes, unsigned *info,
+                             void *stream) {
   if (n > kMax) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_tlas_build, dim3(1), dim3(kThreads), 0, (hipStream_t)stream,
-                     (const float4 *)aabbs, n, (float *)work, (dnode *)nodes, info);
+                     (const float4 *)aabbs, n, (dnode *)nodes, info);
   return (int)hipGetLastError();
 }
-
-extern "C" unsigned rk_tlas_work_floats(unsigned n) { return (n ? n : 1u) * kWork; }
