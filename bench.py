@@ -97,11 +97,13 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
     run(probe)
     dt = max(time.perf_counter() - t, 1e-6)
     n = int(min(len(rays), max(20000, len(probe) / dt * seconds)))
-    # whole passes over the buffer until about `seconds` of CPU work
-    passes = max(1, int(seconds / max(dt / len(probe) * n, 1e-9)))
+    # whole passes over the (prefix of the) buffer until `seconds` of CPU
+    # work have elapsed (the probe's per-ray time is only an estimate)
+    passes = 0
     t = time.perf_counter()
-    for _ in range(passes):
+    while passes == 0 or time.perf_counter() - t < seconds:
         run(rays[:n])
+        passes += 1
     dt = time.perf_counter() - t
     n *= passes
     if rs is not None:
