@@ -254,3 +254,26 @@ def test_rebuild_after_moving_instances():
         assert_same_hits(s, random_rays(s, 100_000, seed=35), "moved")
     finally:
         s.close()
+
+
+def test_path_tracing_through_device_tlas():
+    """Path-traced frames of the instanced scene (graph-replayed) with the
+    host tree, then with the device tree after a rebuild: the same samples
+    give the same image except where a near tie picks another hit."""
+    s = Scene(config=3, detail=0.03, width=96, height=64, spp=4)
+    try:
+        say("device TLAS: path tracing, host tree then device tree")
+        rebuild(s, REND_TLAS_HOST)
+        a, ra = s.render(frames=3, maxdepth=4)
+        try:
+            rebuild(s, REND_TLAS_GPU)
+            b, rb = s.render(frames=3, maxdepth=4)
+        finally:
+            lib().rend_hip_set_tlas_build(REND_TLAS_HOST)
+        same = (a == b).all(axis=2)
+        rel = float(((a - b) ** 2).sum() / max((a ** 2).sum(), 1e-30))
+        say(f"  {same.mean():.5f} of pixels identical, relMSE {rel:.2e}, rays {ra} vs {rb}")
+        assert a.sum() > 0 and same.mean() > 0.999 and rel < 1e-4
+        assert abs(ra - rb) <= 1e-3 * ra
+    finally:
+        s.close()
