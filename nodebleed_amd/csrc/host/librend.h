@@ -85,6 +85,8 @@ struct rl_scene {
 	void                *aabbs_dev;  /* instance world AABBs for the device build */
 	unsigned            *tlas_info;  /* device build result: nodes, error */
 	size_t               aabbs_cap;
+	void                *tlas_stage; /* pinned host staging of the device build */
+	size_t               tlas_stage_cap;
 	unsigned             hazards;
 	struct rl_frame     *frame;
 	struct rl_pt        *pt;

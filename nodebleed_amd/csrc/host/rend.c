@@ -225,6 +225,9 @@ static void scene_del(struct rdata *rd)
 			dfree((void **)&s->insts);
 			dfree((void **)&s->aabbs_dev);
 			dfree((void **)&s->tlas_info);
+			if (s->tlas_stage)
+				hipHostFree(s->tlas_stage);
+			s->tlas_stage = NULL;
 			dfree((void **)&s->nrms);
 			dfree((void **)&s->mtls);
 			dfree((void **)&s->tris);
@@ -498,25 +501,38 @@ static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
 	int r = arena_reserve(s, at + sz, s->blas_bytes);
 	if (r)
 		return r;
-	struct dinst *di = malloc(((size_t)n + 1) * sizeof(*di));
-	if (!di)
-		return rl_fail(REND_ENOMEM, "instance staging");
+	/* pinned staging (instances, boxes, result words), kept per scene, so
+	 * the uploads are plain DMA */
+	size_t ib = ((size_t)n + 1) * sizeof(struct dinst);
+	size_t ab = ((size_t)n + 1) * sizeof(struct aabb);
+	size_t need = ib + ab + 4 * sizeof(unsigned);
+	hipError_t e = hipSuccess;
+	if (need > s->tlas_stage_cap) {
+		if (s->tlas_stage)
+			hipHostFree(s->tlas_stage);
+		s->tlas_stage = NULL;
+		s->tlas_stage_cap = 0;
+		if ((e = hipHostMalloc(&s->tlas_stage, need, hipHostMallocDefault)) != hipSuccess)
+			return rl_hipfail(e, "TLAS staging");
+		s->tlas_stage_cap = need;
+	}
+	struct dinst *di = (struct dinst *)s->tlas_stage;
+	struct aabb *sab = (struct aabb *)((unsigned char *)s->tlas_stage + ib);
+	unsigned *info = (unsigned *)((unsigned char *)s->tlas_stage + ib + ab);
 	for (unsigned i = 0; i < n; i++) {
 		const struct rinst *ri = &rd->insts[i];
 		int b = blas_find(s, ri->triofs);
-		if (b < 0) {
-			free(di);
+		if (b < 0)
 			return rl_fail(REND_ESTATE,
 			  "instance %u references a BLAS not built by rend_prepstatic", i);
-		}
 		memcpy(di[i].inv, ri->globinv, sizeof(di[i].inv));
 		di[i].blas = s->blas_dev[b];
 		di[i].triofs = ri->triofs;
 		di[i].flags = ri->flags;
 	}
-	hipError_t e = hipSuccess;
-	size_t ib = ((size_t)n + 1) * sizeof(*di);
-	size_t ab = ((size_t)n + 1) * sizeof(struct aabb);
+	if (n)
+		memcpy(sab, rd->aabbs, (size_t)n * sizeof(struct aabb));
+	info[0] = info[1] = 0;
 	if (ib > s->insts_cap) {
 		dfree((void **)&s->insts);
 		s->insts_cap = 0;
@@ -535,20 +551,16 @@ static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
 		e = hipMemcpyAsync(s->insts, di, (size_t)n * sizeof(*di), hipMemcpyHostToDevice,
 		  G.stream);
 	if (e == hipSuccess && n)
-		e = hipMemcpyAsync(s->aabbs_dev, rd->aabbs, (size_t)n * sizeof(struct aabb),
+		e = hipMemcpyAsync(s->aabbs_dev, sab, (size_t)n * sizeof(struct aabb),
 		  hipMemcpyHostToDevice, G.stream);
-	if (e != hipSuccess) {
-		free(di);
+	if (e != hipSuccess)
 		return rl_hipfail(e, "device TLAS inputs");
-	}
 	r = rk_tlas_build(s->aabbs_dev, n, s->arena + at, s->tlas_info, G.stream);
-	unsigned info[2] = {0, 0};
 	if (!r)
-		r = (int)hipMemcpyAsync(info, s->tlas_info, sizeof(info), hipMemcpyDeviceToHost,
-		  G.stream);
+		r = (int)hipMemcpyAsync(info, s->tlas_info, 2 * sizeof(unsigned),
+		  hipMemcpyDeviceToHost, G.stream);
 	if (!r)
 		r = (int)hipStreamSynchronize(G.stream);
-	free(di);
 	if (r)
 		return rl_hipfail(r, "device TLAS build");
 	if (info[1] || !info[0] || (size_t)info[0] * sizeof(struct bnode8) > sz)
