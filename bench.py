@@ -378,7 +378,8 @@ def bench_guide(a):
     tp, plain, cp = run(a.guide_frames, False, ref=ref, check_every=every, tag="plain")
     err_p = rel_mse(plain, ref)
     log(f"plain: {a.guide_frames * spf} spp in {tp:.2f} s, relMSE {err_p:.4f}")
-    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, every // 8), stop_at=err_p,
+    tg, guided, cg = run(1 << 30, True, ref=ref, check_every=max(1, min(8, every // 8)),
+                         stop_at=err_p,
                          tmax=3.0 * tp, tag="guided")
     err_g = rel_mse(guided, ref)
     info = (C.c_float * 4)()
@@ -425,7 +426,8 @@ def main():
                          "instanced path tracing; views: 32 uneven cameras in one batched "
                          "launch; guide: equal-time guided vs plain path tracing; tlas: "
                          "per-frame TLAS rebuild, host SAH vs device LBVH")
-    ap.add_argument("--guide-frames", type=int, default=256)
+    ap.add_argument("--guide-frames", type=int, default=1024,
+                    help="plain frames (16 spp each) whose error guided must reach")
     ap.add_argument("--guide-ref-frames", type=int, default=4096)
     ap.add_argument("--guide-spf", type=int, default=16, help="samples per pixel per frame")
     ap.add_argument("--guide-mix", type=float, default=0.5)
