@@ -359,7 +359,10 @@ def bench_guide(a):
             el += time.perf_counter() - t
             samples += spf
             k += 1
-            if check_every and k % check_every == 0:
+            # near the target the error is checked every frame, so the time
+            # reported is that of the first frame that reaches it
+            near = stop_at is not None and curve and curve[-1][2] <= 1.1 * stop_at
+            if check_every and (k % check_every == 0 or near):
                 err = rel_mse(download(k * spf), ref)
                 curve.append((round(el, 3), k * spf, err))
                 log(f"  {tag} {k * spf} spp {el:.2f} s relMSE {err:.4f}")
