@@ -319,3 +319,28 @@ def test_importance_sampled_training_learns_a_known_lobe(guided_scene):
     say(f"loss {loss.value:.3f}; density at the lobe {np.median(p_mu):.2f}, opposite "
         f"{np.median(p_opp):.4f}")
     assert np.median(p_mu) > 3.0 and np.median(p_opp) < 0.05
+
+
+def test_warmup_schedule_counts_adam_steps():
+    """rend_hip_guide_warmup: the first `frames` frames after enabling train
+    with the warm-up step count, later frames with the per-frame count; with
+    training frozen (0 steps per frame) no step is taken, warm-up or not."""
+    s = Scene(config=5, detail=0.1, width=64, height=36, spp=4)
+    L = lib()
+    info = (C.c_float * 4)()
+    try:
+        for warm, want in (((16, 64), 16 * 64 + 4 * 8), ((0, 0), 20 * 8), ((3, 4), 20 * 8)):
+            check(L.rend_hip_guide_enable(s.rd, 0.5, 4, 8, 1e-2, 7), "guide enable")
+            check(L.rend_hip_guide_warmup(s.rd, *warm), "warm-up")
+            say(f"warm-up {warm}: 20 frames")
+            s.render(frames=20, maxdepth=3)
+            check(L.rend_hip_guide_info(s.rd, info), "info")
+            assert int(info[2]) == want, (warm, info[2])
+        check(L.rend_hip_guide_enable(s.rd, 0.5, 4, 0, 1e-2, 7), "guide enable frozen")
+        s.render(frames=3, maxdepth=3)
+        check(L.rend_hip_guide_info(s.rd, info), "info")
+        assert int(info[2]) == 0
+        assert L.rend_hip_guide_warmup(s.rd, 1, 5000) != 0
+    finally:
+        L.rend_hip_guide_disable(s.rd)
+        s.close()
