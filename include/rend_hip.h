@@ -220,6 +220,11 @@ int          rend_hip_guide_mix_host(struct rdata *rd, const float *pn,
  * exponent for which 2^(e-1) <= wsum < 2^e. */
 int          rend_hip_guide_cdf_host(const float *values, unsigned int n, float wsum,
                                      unsigned long long *cdf);
+/* The proposal's direction sampling routine (the device code, compiled for
+ * the host as well) for one mixture record (GD_MIX floats: per lobe weight,
+ * mean xyz, concentration) and three uniforms in [0, 1); needs no device. */
+int          rend_hip_guide_sample_host(const float *mix, float u0, float u1, float u2,
+                                        float dir[3]);
 /* D = A (16x32) * B (32x16), row-major fp32 in, bf16 MFMA inside. */
 int          rend_hip_mfma_check(const float *A, const float *B, float *D);
 

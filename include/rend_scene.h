@@ -50,6 +50,14 @@ int  rs_torus(struct rs_scene *s, unsigned int nu, unsigned int nv, float R,
 int  rs_terrain(struct rs_scene *s, unsigned int nx, unsigned int nz,
                 float x0, float x1, float z0, float z1, float amp, float freq,
                 unsigned int seed, unsigned int mtl);
+/* Indexed triangles: nv vertices (3 floats each), optional vertex normals
+ * (NULL: area-weighted normals of these triangles), ntri index triples into
+ * this block of vertices, and per-triangle material ids (mtls) or one
+ * material for all (mtls == NULL).  Returns 0, or -1 on an index or material
+ * out of range. */
+int  rs_mesh_data(struct rs_scene *s, const float *v, unsigned int nv, const float *n,
+                  const unsigned int *idx, unsigned int ntri, const unsigned int *mtls,
+                  unsigned int mtl);
 /* Quad p0 p1 p2 p3 (counter-clockwise seen from the front) as 2 triangles. */
 int  rs_quad(struct rs_scene *s, const float p0[3], const float p1[3],
              const float p2[3], const float p3[3], unsigned int mtl);
@@ -75,6 +83,7 @@ int  rs_set_rcam(struct rdata *rd, const float eye[3], const float target[3],
                  unsigned int h);
 
 unsigned int rs_tricount(const struct rs_scene *s);
+unsigned int rs_mtlcount(const struct rs_scene *s);
 
 /* Benchmark scenes.  config: 1 sphere-in-box, 2 terrain (1000 x 500 cells),
  * 3 instanced meshes, 5 occluded-lights terrain.  detail < 1 scales triangle

@@ -137,6 +137,8 @@ _SIGS = {
     "rend_hip_guide_mix_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p]),
     "rend_hip_guide_train_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint,
                                             C.POINTER(C.c_float)]),
+    "rend_hip_guide_sample_host": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                             C.c_void_p]),
     "rend_hip_mfma_check": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "rs_new": (C.c_void_p, []),
     "rs_free": (None, [C.c_void_p]),
@@ -162,6 +164,9 @@ _SIGS = {
                               C.c_float, C.c_float, C.c_float, C.c_uint, C.c_uint]),
     "rs_background": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
     "rs_tricount": (C.c_uint, [C.c_void_p]),
+    "rs_mtlcount": (C.c_uint, [C.c_void_p]),
+    "rs_mesh_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p,
+                               C.c_uint, C.c_void_p, C.c_uint]),
     "rs_noise": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_uint]),
     "rs_instance_glob": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_float)]),
     "rs_rdata_view": (None, [C.c_void_p, C.POINTER(RsView)]),
@@ -229,6 +234,7 @@ class Scene:
         self.rd = L.rs_rdata_new()
         check(L.rs_commit(self.rs, self.rd, width, height, spp), "rs_commit")
         self.ntris = L.rs_tricount(self.rs)
+        self.nmtls = L.rs_mtlcount(self.rs)
 
     @classmethod
     def from_builder(cls, fill, width=64, height=64, spp=1):

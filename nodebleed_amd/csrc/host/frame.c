@@ -13,6 +13,7 @@
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -33,6 +34,10 @@ int rk_shade_normals(const void *hits, const void *insts, const void *nrms,
                      unsigned rh, unsigned width, unsigned spp,
                      float invsamples, float *acc, unsigned *buf,
                      void *stream);
+int rk_shade_normals_ref(const void *hits, const void *insts, const void *nrms,
+                         const void *mtls, const float bg[3], unsigned bszx, unsigned bszy,
+                         unsigned blkcntx, unsigned nblk, unsigned width, unsigned height,
+                         float invsamples, float *acc, unsigned *buf, void *stream);
 int rk_fill_f32(float *p, unsigned long long n, float v, void *stream);
 int rk_diffuse_rays(const void *prim, const void *hits, const void *insts,
                     const void *tris, unsigned n, unsigned seed, void *out,
@@ -139,7 +144,18 @@ static int frame_buffers(struct rl_scene *s, size_t nrays)
 	return 0;
 }
 
-/* Normal-tint frame (the reference's active rend_render). */
+static int rsqrt_table_ready(hipStream_t st);
+
+static int is_pow2(unsigned v) { return v && !(v & (v - 1)); }
+
+/* Normal-tint frame: the reference's active rend_render (rend.c:3603-3708).
+ * Like the reference it traces one primary ray per pixel of the block grid
+ * per frame and divides the accumulator by spp + samples.  With a pinhole
+ * camera and blocks made of 4x2 packets in a Morton square (or 2:1) the rays
+ * are the reference's own (make_camray_pckt with its xorshift128+ streams
+ * and RSQRTPS), so the frame equals the reference's single-ray shading of
+ * its primaries bit for bit; otherwise (thin lens, other block shapes) the
+ * rays come from k_camrays' jittered, exactly normalised camera. */
 static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
                           unsigned *rays_out)
 {
@@ -148,23 +164,36 @@ static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
 	int r = rl_acc_ready(s, st);
 	if (r)
 		return r;
-	unsigned spp = rd->spp ? rd->spp : 1;
-	size_t nrays = (size_t)rw * rh * spp;
+	const unsigned bx = rl_blkszx(), by = rl_blkszy();
+	const int refrays = !(rd->cam.tanfangle > 0.0f) && bx % 4 == 0 && by % 2 == 0 &&
+	  is_pow2(by / 2) && (bx / 4 == by / 2 || bx / 4 == by);
+	size_t nrays = (size_t)rw * rh;
 	if ((r = frame_buffers(s, nrays)))
 		return r;
 	struct CamArgs ca;
 	rl_camargs(&rd->cam, &ca);
 	struct rl_frame *f = s->frame;
-	if ((r = rk_camrays(&ca, rw, rh, spp, 0x2545f491u, rd->samples, f->rays,
-	  st)))
+	const unsigned blkcntx = rd->width / bx, nblk = blkcntx * (rd->height / by);
+	if (refrays) {
+		if ((r = rsqrt_table_ready(st)))
+			return r;
+		if ((r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, rd->samples, f->rays, st)))
+			return rl_hipfail(r, "reference camera rays");
+	} else if ((r = rk_camrays(&ca, rw, rh, 1, 0x2545f491u, rd->samples, f->rays, st))) {
 		return rl_hipfail(r, "camera rays");
+	}
 	if ((r = rl_trace_async(s, f->rays, f->hits, NULL, (unsigned)nrays, 1, 0,
 	  st)))
 		return r;
 	float bg[3] = {rd->bgcol.x, rd->bgcol.y, rd->bgcol.z};
 	float inv = 1.0f / (float)(rd->spp + rd->samples);
-	if ((r = rk_shade_normals(f->hits, s->insts, s->nrms, s->mtls, bg, rw, rh,
-	  rd->width, spp, inv, s->acc, s->buf, st)))
+	if (refrays)
+		r = rk_shade_normals_ref(f->hits, s->insts, s->nrms, s->mtls, bg, bx, by, blkcntx,
+		  nblk, rd->width, rd->height, inv, s->acc, s->buf, st);
+	else
+		r = rk_shade_normals(f->hits, s->insts, s->nrms, s->mtls, bg, rw, rh,
+		  rd->width, 1, inv, s->acc, s->buf, st);
+	if (r)
 		return rl_hipfail(r, "shade");
 	f->frame++;
 	*rays_out = (unsigned)nrays;
@@ -185,8 +214,16 @@ int rend_render(void *d)
 
 	unsigned rays = 0;
 	int r = rend_hip_render_frame(rd, &rays);
-	if (r)
+	if (r) {
+		/* The reference's thread entry returns 0 whatever happens; the
+		 * failure stays readable through rend_hip_lasterror() (sticky)
+		 * and is reported on stderr once per process. */
+		static int noted;
+		if (!__atomic_exchange_n(&noted, 1, __ATOMIC_SEQ_CST))
+			fprintf(stderr, "librend: rend_render failed (%d): %s\n", r,
+			  rend_hip_errstr());
 		return 0;
+	}
 	__atomic_fetch_add(&rd->rays, rays, __ATOMIC_SEQ_CST);
 	return 0;
 }
@@ -233,6 +270,13 @@ int rend_hip_render_frame(struct rdata *rd, unsigned *rays)
 		if (e != hipSuccess)
 			return rl_hipfail(e, "colour buffer download");
 	}
+	if (rd->acc) {
+		/* the reference accumulates into rd->acc itself (rend.c:3571) */
+		e = hipMemcpy(rd->acc, s->acc, (size_t)rd->width * rd->height *
+		  3 * sizeof(float), hipMemcpyDeviceToHost);
+		if (e != hipSuccess)
+			return rl_hipfail(e, "accumulator download");
+	}
 	if (rays)
 		*rays = n;
 	return 0;
@@ -255,6 +299,23 @@ int rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays, unsigned int w,
 static const unsigned g_rsqrt_tab[4096] = RSQRT_TABLE_INIT;
 static int g_rsqrt_dev = -1;  /* device the table was uploaded to */
 
+/* Host RSQRTPS table in the camera kernel's constant memory (once per
+ * device). */
+static int rsqrt_table_ready(hipStream_t st)
+{
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e)
+		return rl_hipfail(e, "device");
+	if (g_rsqrt_dev != dev) {
+		int r = rk_camrays_ref_setup(g_rsqrt_tab, st);
+		if (r)
+			return rl_hipfail(r, "rsqrt table");
+		g_rsqrt_dev = dev;
+	}
+	return 0;
+}
+
 unsigned long long rend_hip_camrays_ref_count(struct rdata *rd)
 {
 	const unsigned bx = rl_blkszx(), by = rl_blkszy();
@@ -275,15 +336,8 @@ int rend_hip_gen_camrays_ref(struct rdata *rd, struct rray *d_rays, unsigned int
 	if (rd->cam.tanfangle > 0.0f)
 		return rl_fail(REND_EINVAL, "reference-exact rays need a pinhole camera");
 	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
-	int dev = 0;
-	hipError_t e = hipGetDevice(&dev);
-	if (e)
-		return rl_hipfail(e, "device");
-	if (g_rsqrt_dev != dev) {
-		if ((r = rk_camrays_ref_setup(g_rsqrt_tab, st)))
-			return rl_hipfail(r, "rsqrt table");
-		g_rsqrt_dev = dev;
-	}
+	if ((r = rsqrt_table_ready(st)))
+		return r;
 	struct CamArgs ca;
 	rl_camargs(&rd->cam, &ca);
 	const unsigned blkcntx = rd->width / bx, nblk = blkcntx * (rd->height / by);

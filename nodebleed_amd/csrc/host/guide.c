@@ -327,6 +327,13 @@ int rend_hip_guide_info(struct rdata *rd, float out[4])
 	return 0;
 }
 
+/* Training records a batch of cap paths can write: one per guided vertex
+ * (depth < g->depth) of every stride-th path. */
+static size_t rec_capacity(const struct rl_guide *g, size_t cap)
+{
+	return (cap / g->stride + 1) * (g->depth ? g->depth : 1) + 64;
+}
+
 /* Fills the guide fields of the frame constants. */
 void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 {
@@ -337,7 +344,7 @@ void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 	memcpy(a->gmin, g->gmin, sizeof(a->gmin));
 	memcpy(a->ginv, g->ginv, sizeof(a->ginv));
 	a->train_stride = g->steps ? g->stride : 0;
-	a->train_cap = (unsigned)(cap / g->stride + 64);
+	a->train_cap = (unsigned)rec_capacity(g, cap);
 	a->guide_depth = g->depth;
 	a->guide_share = 1;
 	a->train_bsdf = train_bsdf(g);
@@ -348,7 +355,7 @@ void rl_guide_args(struct rl_scene *s, struct PtArgs *a, size_t cap)
 int rl_guide_buffers(struct rl_scene *s, size_t cap)
 {
 	struct rl_guide *g = s->guide;
-	size_t rc = cap / g->stride + 64;
+	size_t rc = rec_capacity(g, cap);
 	hipError_t e;
 	if (cap > g->cap) {
 		dfree((void **)&g->mix);

@@ -30,6 +30,13 @@ struct rl_global {
 	unsigned long long *stats;       /* 4 counters of the counting variant */
 	void               *spill;       /* traversal stack spill area */
 	size_t              spill_bytes;
+	/* Every traversal launch and path-tracing frame shares the control
+	 * ring and the spill area, so launches on different streams are put
+	 * in one order: a launch on stream B waits for an event recorded after
+	 * the last launch on stream A. */
+	hipEvent_t          order_ev;
+	hipStream_t         order_stream;
+	int                 order_pending;
 };
 
 struct rl_frame;  /* frame.c */
@@ -88,6 +95,7 @@ struct rl_scene {
 	void                *tlas_stage; /* pinned host staging of the device build */
 	size_t               tlas_stage_cap;
 	unsigned             hazards;
+	unsigned long long   gen;        /* bumped by every upload of scene data */
 	struct rl_frame     *frame;
 	struct rl_pt        *pt;
 	struct rl_guide     *guide;
@@ -104,6 +112,8 @@ int              rl_trace_async(struct rl_scene *s, const void *d_rays,
                                 void *d_hits, void *d_occ, unsigned n,
                                 int closest, int count, hipStream_t st);
 int              rl_check_errors(hipStream_t st);
+int              rl_stream_order(hipStream_t st);
+int              rl_stream_mark(hipStream_t st);
 unsigned         rl_blkszx(void);
 unsigned         rl_blkszy(void);
 void             rl_frame_release(struct rl_scene *s);

@@ -49,6 +49,7 @@ struct rl_pt {
 	float    *lcdf;
 	unsigned  nlights;
 	int       lights_ok;
+	unsigned long long lights_gen;  /* scene upload generation they follow */
 };
 
 static struct pt_params {
@@ -74,11 +75,15 @@ static void dfree(void **p)
 	*p = NULL;
 }
 
+static void graph_drop(void);
+
 void rl_pt_release(struct rl_scene *s)
 {
 	struct rl_pt *t = s->pt;
 	if (!t)
 		return;
+	/* the cached frame graph may point at this scene's buffers */
+	graph_drop();
 	dfree(&t->paths);
 	dfree(&t->rays[0]);
 	dfree(&t->rays[1]);
@@ -107,6 +112,7 @@ static int pt_buffers(struct rl_scene *s, size_t cap)
 	struct rl_pt *t = s->pt;
 	if (cap <= t->cap)
 		return 0;
+	graph_drop();
 	dfree(&t->paths);
 	dfree(&t->rays[0]);
 	dfree(&t->rays[1]);
@@ -138,12 +144,15 @@ static int pt_buffers(struct rl_scene *s, size_t cap)
 }
 
 /* Emissive triangles in world space with an area * luminance CDF.  World
- * vertices come from inverting the 3x4 globinv in double. */
+ * vertices come from inverting the 3x4 globinv in double.  Rebuilt after
+ * every upload of scene data (rend_prepstatic, rend_prepdynamic): moving an
+ * emissive instance moves its triangles here too. */
 static int pt_lights(struct rl_scene *s)
 {
 	struct rl_pt *t = s->pt;
-	if (t->lights_ok)
+	if (t->lights_ok && t->lights_gen == s->gen)
 		return 0;
+	graph_drop();
 	struct rdata *rd = s->rd;
 	unsigned cap = 64, n = 0;
 	struct PtLight *L = malloc(cap * sizeof(*L));
@@ -250,6 +259,7 @@ static int pt_lights(struct rl_scene *s)
 	free(cdf);
 	t->nlights = n;
 	t->lights_ok = 1;
+	t->lights_gen = s->gen;
 	return 0;
 }
 
@@ -347,10 +357,16 @@ static unsigned long long pt_key(const struct rl_scene *s, const struct PtArgs *
 		c.sample_base = 0;
 		MIX(&c, sizeof(c));
 	}
-	const void *ptrs[7] = {acc, st, s->arena, s->insts, s->pt->paths, s->pt->views, G.spill};
+	/* every device pointer the captured kernels receive */
+	const struct rl_pt *t = s->pt;
+	const void *ptrs[] = {s, acc, st, s->arena, s->insts, s->tris, s->nrms, s->mtls,
+	  t->paths, t->views, t->rays[0], t->rays[1], t->q[0], t->q[1], t->hits, t->srays,
+	  t->spath, t->occ, t->scontrib, t->counts, t->lights, t->lcdf, G.ctl, G.diag,
+	  G.spill};
 	MIX(ptrs, sizeof(ptrs));
-	unsigned long long misc[7] = {s->tlas, s->pt->batch, G.refill, (unsigned long long)G.rcp_ieee,
-	  P.shade_mult, (unsigned long long)G.variant, G.grid_mult};
+	unsigned long long misc[] = {s->tlas, t->batch, G.refill, (unsigned long long)G.rcp_ieee,
+	  P.shade_mult, (unsigned long long)G.variant, G.grid_mult, s->gen, t->lights_gen,
+	  G.spill_bytes, t->cap};
 	MIX(misc, sizeof(misc));
 #undef MIX
 	return h;
@@ -398,6 +414,8 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 			return rl_hipfail(e, "views");
 		t->views_cap = nv;
 	}
+	if ((r = rl_stream_order(st)))
+		return r;
 	hipError_t e = hipMemcpyAsync(t->views, views, nv * sizeof(*views),
 	  hipMemcpyHostToDevice, st);
 	if (e)
@@ -446,6 +464,8 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 	} else if ((r = pt_launches(s, &a, total, acc, st))) {
 		return r;
 	}
+	if ((r = rl_stream_mark(st)))
+		return r;
 	if (rays) {
 		unsigned long long n = 0;
 		e = hipMemcpyAsync(&n, &t->counts[4], sizeof(n), hipMemcpyDeviceToHost, st);

@@ -95,6 +95,10 @@ int rl_device(void)
 		return rl_hipfail(e, "control words");
 	}
 	G.diag = G.ctl + RL_CTL_SLOTS * RK_CTL_WORDS;
+	if ((e = hipEventCreateWithFlags(&G.order_ev, hipEventDisableTiming)) != hipSuccess) {
+		pthread_mutex_unlock(&g_lock);
+		return rl_hipfail(e, "launch order event");
+	}
 	G.ready = 1;
 	pthread_mutex_unlock(&g_lock);
 	return 0;
@@ -174,6 +178,37 @@ int rl_trace_grid(int closest, int count, unsigned *grid)
 		G.spill_bytes = need;
 	}
 	*grid = g;
+	return 0;
+}
+
+/* Launch order across streams (see struct rl_global): before work that
+ * uses the shared control ring or spill area is queued on st, st waits for
+ * the last such work queued on another stream.  A stream under capture is
+ * left alone: the capturing caller ordered the stream before it began. */
+static int capturing(hipStream_t st)
+{
+	hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+	return hipStreamIsCapturing(st, &cs) == hipSuccess &&
+	  cs != hipStreamCaptureStatusNone;
+}
+
+int rl_stream_order(hipStream_t st)
+{
+	if (!G.order_pending || G.order_stream == st || capturing(st))
+		return 0;
+	hipError_t e = hipStreamWaitEvent(st, G.order_ev, 0);
+	return e == hipSuccess ? 0 : rl_hipfail(e, "launch order");
+}
+
+int rl_stream_mark(hipStream_t st)
+{
+	if (capturing(st))
+		return 0;
+	hipError_t e = hipEventRecord(G.order_ev, st);
+	if (e != hipSuccess)
+		return rl_hipfail(e, "launch order mark");
+	G.order_stream = st;
+	G.order_pending = 1;
 	return 0;
 }
 
@@ -484,6 +519,7 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 		return rl_hipfail(e, "shading data upload");
 	s->static_ok = 1;
 	s->dynamic_ok = 0;
+	s->gen++;
 	return 0;
 }
 
@@ -571,6 +607,7 @@ static int upload_dynamic_dev(struct rdata *rd, struct rl_scene *s)
 	s->arena_used = at + s->tlas_bytes;
 	s->ninsts = n;
 	s->dynamic_ok = 1;
+	s->gen++;
 	return 0;
 }
 
@@ -640,6 +677,7 @@ static int upload_dynamic(struct rdata *rd, struct rl_scene *s)
 	s->tlas_bytes = (size_t)sz;
 	s->ninsts = rd->instcnt;
 	s->dynamic_ok = 1;
+	s->gen++;
 	return 0;
 }
 
@@ -808,6 +846,8 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	int r = rl_trace_grid(closest, count, &grid);
 	if (r)
 		return r;
+	if ((r = rl_stream_order(st)))
+		return r;
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
 	  d_occ, n, G.ctl + RK_CTL_WORDS * slot, G.spill, count ? G.stats : NULL,
@@ -815,7 +855,7 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;
-	return 0;
+	return rl_stream_mark(st);
 }
 
 static int check_ready(struct rdata *rd, struct rl_scene **out)
@@ -905,6 +945,8 @@ int rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
 		return r;
 	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
 	if (stats) {
+		if ((r = rl_stream_order(st)))
+			return r;
 		hipError_t e = hipMemsetAsync(G.stats, 0,
 		  4 * sizeof(unsigned long long), st);
 		if (e != hipSuccess)

@@ -253,6 +253,41 @@ static void smooth_normals(struct rs_scene *s, unsigned v0, unsigned t0)
 	}
 }
 
+int rs_mesh_data(struct rs_scene *s, const float *v, unsigned int nv, const float *n,
+                 const unsigned int *idx, unsigned int ntri, const unsigned int *mtls,
+                 unsigned int mtl)
+{
+	if (!s->open || !v || !idx)
+		return -1;
+	for (unsigned t = 0; t < 3 * ntri; t++)
+		if (idx[t] >= nv)
+			return -1;
+	for (unsigned t = 0; mtls && t < ntri; t++)
+		if (mtls[t] >= s->nmtl)
+			return -1;
+	if (!mtls && ntri && mtl >= s->nmtl)
+		return -1;
+	const unsigned v0 = s->cur.nv, t0 = s->cur.ntri;
+	for (unsigned i = 0; i < nv; i++) {
+		if (add_vert(s, v[3 * i], v[3 * i + 1], v[3 * i + 2]) < 0)
+			return -1;
+		if (n)
+			memcpy(&s->cur.n[3 * (v0 + i)], &n[3 * i], 3 * sizeof(float));
+	}
+	for (unsigned t = 0; t < ntri; t++)
+		if (add_tri(s, v0 + idx[3 * t], v0 + idx[3 * t + 1], v0 + idx[3 * t + 2],
+		  mtls ? mtls[t] : mtl) < 0)
+			return -1;
+	if (!n)
+		smooth_normals(s, v0, t0);
+	return 0;
+}
+
+unsigned int rs_mtlcount(const struct rs_scene *s)
+{
+	return s->nmtl;
+}
+
 int rs_mesh_end(struct rs_scene *s)
 {
 	if (!s->open)

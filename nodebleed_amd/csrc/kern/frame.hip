@@ -6,8 +6,13 @@
 //               uses the approximate RSQRTPS) and jitter from a counter-based
 //               hash instead of xorshift128+ packets, so images agree with the
 //               reference statistically, not bit for bit.
-//   k_shade_normals  trace_rays/calc_nrm (rend.c:3404-3440, 3109-3116):
-//               colour (n * 0.5 + 0.5) * albedo, background on a miss.
+//   k_camrays_ref  the reference's own pinhole primary rays, bit for bit
+//               (xorshift128+ packets, host RSQRTPS table): what rend_render
+//               traces for a pinhole camera.
+//   k_shade_normals, k_shade_normals_ref  trace_rays/calc_nrm
+//               (rend.c:3404-3440, 3109-3116): colour (n * 0.5 + 0.5) *
+//               albedo, background on a miss, for pixel-major rays or for
+//               rays in the reference's block/packet order.
 //   accumulate  accum_pckt (rend.c:3565-3580): acc += colour, ARGB8 from
 //               acc / (spp + samples) with the reference's 8-bit wrap.
 
@@ -103,6 +108,51 @@ __device__ __forceinline__ uint32_t pack8(float a) {
   return (uint32_t)(long long)(255.0f * a) & 0xffu;
 }
 
+// The reference's normal tint of one hit (trace_rays/calc_nrm,
+// rend.c:3419-3447, 3109-3116): (unit(M^-T n) + 1) * 0.5 * albedo, the
+// background on a miss; same operations, same order, no fusion.
+__device__ __forceinline__ void shade_hit(const rhit &h, const dinst *__restrict__ insts,
+                                          const RNrm *__restrict__ nrms,
+                                          const RMtl *__restrict__ mtls, float bgr,
+                                          float bgg, float bgb, float &r, float &g,
+                                          float &b) {
+  r = bgr; g = bgg; b = bgb;
+  if (h.t < 3.402823466e+38f) {
+    const uint32_t inst = h.id & 0xfffu, tri = h.id >> 12;
+    const dinst &in = insts[inst];
+    const RNrm &rn = nrms[in.triofs + tri];
+    const float u = h.u, v = h.v, w = 1.0f - u - v;
+    float nx = rn.n1[0] * u + (rn.n2[0] * v + rn.n0[0] * w);
+    float ny = rn.n1[1] * u + (rn.n2[1] * v + rn.n0[1] * w);
+    float nz = rn.n1[2] * u + (rn.n2[2] * v + rn.n0[2] * w);
+    // inverse transpose of the 3x3 part of globinv (rend.c:3431-3436)
+    const float *m = in.inv;
+    const float tx = m[0] * nx + m[4] * ny + m[8] * nz;
+    const float ty = m[1] * nx + m[5] * ny + m[9] * nz;
+    const float tz = m[2] * nx + m[6] * ny + m[10] * nz;
+    const float il = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
+    nx = tx * il; ny = ty * il; nz = tz * il;
+    const RMtl &mt = mtls[rn.mtlid];
+    r = (nx + 1.0f) * 0.5f * mt.col[0];
+    g = (ny + 1.0f) * 0.5f * mt.col[1];
+    b = (nz + 1.0f) * 0.5f * mt.col[2];
+  }
+}
+
+// accum_pckt (rend.c:3565-3580) for one pixel.
+__device__ __forceinline__ void accum_px(float *__restrict__ acc, uint32_t *__restrict__ buf,
+                                         size_t o, float cr, float cg, float cb,
+                                         float invsamples) {
+  const float ar = acc[3 * o + 0] + cr;
+  const float ag = acc[3 * o + 1] + cg;
+  const float ab = acc[3 * o + 2] + cb;
+  acc[3 * o + 0] = ar;
+  acc[3 * o + 1] = ag;
+  acc[3 * o + 2] = ab;
+  buf[o] = 0xff000000u | pack8(ar * invsamples) << 16 |
+           pack8(ag * invsamples) << 8 | pack8(ab * invsamples);
+}
+
 // Shade spp hits per pixel with the reference's normal tint and accumulate.
 __global__ void k_shade_normals(const rhit *__restrict__ hits,
                                 const dinst *__restrict__ insts,
@@ -115,40 +165,12 @@ __global__ void k_shade_normals(const rhit *__restrict__ hits,
   if (p >= rw * rh) return;
   float cr = 0.0f, cg = 0.0f, cb = 0.0f;
   for (uint32_t s = 0; s < spp; s++) {
-    const rhit h = hits[(size_t)p * spp + s];
-    float r = bgr, g = bgg, b = bgb;
-    if (h.t < 3.402823466e+38f) {
-      const uint32_t inst = h.id & 0xfffu, tri = h.id >> 12;
-      const dinst &in = insts[inst];
-      const RNrm &rn = nrms[in.triofs + tri];
-      const float u = h.u, v = h.v, w = 1.0f - u - v;
-      float nx = rn.n1[0] * u + (rn.n2[0] * v + rn.n0[0] * w);
-      float ny = rn.n1[1] * u + (rn.n2[1] * v + rn.n0[1] * w);
-      float nz = rn.n1[2] * u + (rn.n2[2] * v + rn.n0[2] * w);
-      // inverse transpose of the 3x3 part of globinv (rend.c:3431-3436)
-      const float *m = in.inv;
-      const float tx = m[0] * nx + m[4] * ny + m[8] * nz;
-      const float ty = m[1] * nx + m[5] * ny + m[9] * nz;
-      const float tz = m[2] * nx + m[6] * ny + m[10] * nz;
-      const float il = 1.0f / sqrtf(tx * tx + ty * ty + tz * tz);
-      nx = tx * il; ny = ty * il; nz = tz * il;
-      const RMtl &mt = mtls[rn.mtlid];
-      r = (nx + 1.0f) * 0.5f * mt.col[0];
-      g = (ny + 1.0f) * 0.5f * mt.col[1];
-      b = (nz + 1.0f) * 0.5f * mt.col[2];
-    }
+    float r, g, b;
+    shade_hit(hits[(size_t)p * spp + s], insts, nrms, mtls, bgr, bgg, bgb, r, g, b);
     cr += r; cg += g; cb += b;
   }
   const uint32_t x = p % rw, y = p / rw;
-  const size_t o = (size_t)y * width + x;
-  const float ar = acc[3 * o + 0] + cr;
-  const float ag = acc[3 * o + 1] + cg;
-  const float ab = acc[3 * o + 2] + cb;
-  acc[3 * o + 0] = ar;
-  acc[3 * o + 1] = ag;
-  acc[3 * o + 2] = ab;
-  buf[o] = 0xff000000u | pack8(ar * invsamples) << 16 |
-           pack8(ag * invsamples) << 8 | pack8(ab * invsamples);
+  accum_px(acc, buf, (size_t)y * width + x, cr, cg, cb, invsamples);
 }
 
 struct RTri {  // rend.h:64-71
@@ -340,6 +362,32 @@ __global__ void k_camrays_ref(CamArgs c, uint32_t bszx, uint32_t bszy, uint32_t 
   }
 }
 
+// Normal tint of the hits of the reference's primary rays (k_camrays_ref
+// order: ray = (block * packets + packet) * 8 + lane, packets in Morton
+// order of 4 x 2 pixels), accumulated as accum_pckt does.  Each pixel of the
+// block grid has exactly one ray when the packet grid of a block is 2^a x 2^a
+// or 2^(a+1) x 2^a (checked by the host); a pixel outside width x height is
+// skipped.
+__global__ void k_shade_normals_ref(const rhit *__restrict__ hits,
+                                    const dinst *__restrict__ insts,
+                                    const RNrm *__restrict__ nrms,
+                                    const RMtl *__restrict__ mtls, float bgr, float bgg,
+                                    float bgb, uint32_t bszx, uint32_t bszy, uint32_t blkcntx,
+                                    uint32_t npk, uint32_t n, uint32_t width, uint32_t height,
+                                    float invsamples, float *__restrict__ acc,
+                                    uint32_t *__restrict__ buf) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t pk = i >> 3, lane = i & 7u;
+  const uint32_t blk = pk / npk, k = pk % npk;
+  const uint32_t x = (blk % blkcntx) * bszx + mort_dec(k) * 4u + (lane & 3u);
+  const uint32_t y = (blk / blkcntx) * bszy + mort_dec(k >> 1) * 2u + (lane >> 2);
+  if (x >= width || y >= height) return;
+  float r, g, b;
+  shade_hit(hits[i], insts, nrms, mtls, bgr, bgg, bgb, r, g, b);
+  accum_px(acc, buf, (size_t)y * width + x, r, g, b, invsamples);
+}
+
 extern "C" int rk_camrays_ref_setup(const unsigned *tab, void *stream) {
   return (int)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rsqrt_tab), tab, 4096 * sizeof(uint32_t), 0,
                                      hipMemcpyHostToDevice, (hipStream_t)stream);
@@ -400,5 +448,19 @@ extern "C" int rk_fill_f32(float *p, unsigned long long n, float v, void *stream
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, p, (size_t)n, v);
+  return (int)hipGetLastError();
+}
+
+extern "C" int rk_shade_normals_ref(const void *hits, const void *insts, const void *nrms,
+                                    const void *mtls, const float bg[3], unsigned bszx,
+                                    unsigned bszy, unsigned blkcntx, unsigned nblk,
+                                    unsigned width, unsigned height, float invsamples,
+                                    float *acc, unsigned *buf, void *stream) {
+  const uint32_t npk = bszx * bszy / 8u, n = nblk * npk * 8u;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_shade_normals_ref, dim3((n + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, (const rhit *)hits, (const dinst *)insts,
+                     (const RNrm *)nrms, (const RMtl *)mtls, bg[0], bg[1], bg[2], bszx, bszy,
+                     blkcntx, npk, n, width, height, invsamples, acc, buf);
   return (int)hipGetLastError();
 }

@@ -1202,3 +1202,11 @@ extern "C" int rk_mfma_check(const float *A, const float *B, float *D, void *str
   hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
   return (int)hipGetLastError();
 }
+
+// The sampling routine of the guided vertices, evaluated on the host.
+extern "C" int rend_hip_guide_sample_host(const float *mix, float u0, float u1, float u2,
+                                          float dir[3]) {
+  if (!mix || !dir) return -3;  // REND_EINVAL
+  gd_sample(mix, u0, u1, u2, dir[0], dir[1], dir[2]);
+  return 0;
+}

@@ -52,9 +52,12 @@ __device__ __forceinline__ float gd_pdf(const float *mix, float dx, float dy, fl
   return p;
 }
 
-// Lobe chosen by u0; direction about its mean (Jakob 2012).
-__device__ __forceinline__ void gd_sample(const float *mix, float u0, float u1, float u2,
-                                          float &dx, float &dy, float &dz) {
+// Lobe chosen by u0; direction about its mean (Jakob 2012).  u1 is in [0, 1),
+// so v = 1 - u1 is in (0, 1] and log(v + (1 - v) e^{-2k}) is finite even when
+// e^{-2k} underflows to 0 (k > ~52); v has the distribution of u1.
+__host__ __device__ __forceinline__ void gd_sample(const float *mix, float u0, float u1,
+                                                   float u2, float &dx, float &dy,
+                                                   float &dz) {
   int j = 0;
   float acc = mix[0];
   while (j < GD_LOBES - 1 && u0 >= acc) {
@@ -63,8 +66,9 @@ __device__ __forceinline__ void gd_sample(const float *mix, float u0, float u1, 
   }
   const float mx = mix[5 * j + 1], my = mix[5 * j + 2], mz = mix[5 * j + 3];
   const float k = mix[5 * j + 4];
-  // w = 1 + log(u1 + (1 - u1) e^{-2k}) / k
-  const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+  // w = 1 + log(v + (1 - v) e^{-2k}) / k, v = 1 - u1 in (0, 1]
+  const float v = 1.0f - u1;
+  const float w = 1.0f + logf(v + (1.0f - v) * expf(-2.0f * k)) / k;
   const float sw = sqrtf(fmaxf(0.0f, 1.0f - w * w));
   float sn, cs;
   sincosf(6.28318531f * u2, &sn, &cs);

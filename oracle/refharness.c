@@ -9,6 +9,8 @@
  *              rend_prepdynamic (rend.c:3000-3090, main.c:117-157)
  *   ref_trace: intersect_tlas per ray (rend.c:1976), t starts at tmax
  *   ref_occluded: intersect_any_tlas per ray (rend.c:2883)
+ *   ref_render / ref_render_single: whole frames of rend_render
+ *              (rend.c:3603-3708) and of its single-ray shading path
  * The instance AABB step restates upd_rinsts (main.c:117-157) because main.c
  * needs SDL; it calls the reference's own mulpos.
  */
@@ -81,6 +83,8 @@ struct rdata *ref_scene(const struct rtri *tris, unsigned ntris,
 
 void ref_scene_free(struct rdata *rd)
 {
+	free(rd->buf);
+	rd->buf = NULL;
 	rend_release(rd);
 	free(rd);
 }
@@ -196,4 +200,121 @@ void ref_camrays(struct rdata *rd, unsigned bszx, unsigned bszy, unsigned sample
 float ref_rsqrtps(float x)
 {
 	return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x)));
+}
+
+/* ---- the reference's own frames (rend_render, rend.c:3603-3708) ----------
+ *
+ * ref_frame_setup copies what main.c's cpy_rdata/upd_rcam give rend_render
+ * (materials, normals, camera, background) from an rdata that this
+ * repository's scene builder filled (identical struct layouts), allocates the
+ * accumulator (rend_resaccum) and an application colour buffer.
+ * ref_render runs `frames` frames of the reference's active rend_render on
+ * `threads` threads with main.c's frame protocol (main.c:407-426).
+ * ref_render_single walks the same blocks and packets with the same random
+ * streams but traces every packet with the reference's single-ray
+ * trace_rays (rend.c:3404) before accum_pckt (rend.c:3565): the frame
+ * rend_render would produce if no packet were coherent. */
+void trace_rays(struct vec3 *col, __m256 ox8, __m256 oy8, __m256 oz8,
+                __m256 dx8, __m256 dy8, __m256 dz8, struct rdata *rd,
+                unsigned int *rays);
+void accum_pckt(struct vec3 *acc, unsigned int *buf, struct vec3 *col,
+                unsigned int px, unsigned int py, float *xofs, float *yofs,
+                unsigned int w, float invsamples);
+int rend_render(void *d);
+
+void ref_frame_setup(struct rdata *rd, const struct rdata *src, unsigned nmtls,
+                     unsigned ntris)
+{
+	free(rd->mtls);
+	rd->mtls = aligned_alloc(64, ((size_t)(nmtls ? nmtls : 1) * sizeof(*rd->mtls) + 63) &
+	  ~(size_t)63);
+	memcpy(rd->mtls, src->mtls, (size_t)nmtls * sizeof(*rd->mtls));
+	memcpy(rd->nrms, src->nrms, (size_t)ntris * sizeof(*rd->nrms));
+	rd->cam = src->cam;
+	rd->bgcol = src->bgcol;
+	rd->spp = src->spp;
+	rend_resaccum(rd, src->width, src->height);
+	free(rd->buf);
+	rd->buf = calloc((size_t)src->width * src->height, sizeof(unsigned));
+	rd->samples = 0;
+}
+
+static void *frame_thread(void *rd)
+{
+	rend_render(rd);
+	return NULL;
+}
+
+static void copy_out(const struct rdata *rd, unsigned *buf, float *acc)
+{
+	size_t n = (size_t)rd->width * rd->height;
+	memcpy(buf, rd->buf, n * sizeof(unsigned));
+	memcpy(acc, rd->acc, n * sizeof(struct vec3));
+}
+
+unsigned ref_render(struct rdata *rd, unsigned bszx, unsigned bszy, unsigned frames,
+                    int threads, unsigned *buf, float *acc)
+{
+	rend_staticinit(bszx, bszy);
+	rend_clraccum(rd);
+	memset(rd->buf, 0, (size_t)rd->width * rd->height * sizeof(unsigned));
+	unsigned rays = 0;
+	pthread_t th[64];
+	if (threads < 1)
+		threads = 1;
+	if (threads > 64)
+		threads = 64;
+	for (unsigned f = 0; f < frames; f++) {
+		rd->blknum = 0;
+		rd->rays = 0;
+		int started = 0;
+		for (int i = 0; i < threads; i++)
+			if (!pthread_create(&th[i], NULL, frame_thread, rd))
+				started++;
+			else
+				break;
+		if (!started)
+			rend_render(rd);
+		for (int i = 0; i < started; i++)
+			pthread_join(th[i], NULL);
+		rays += rd->rays;
+		rd->samples += rd->spp;
+	}
+	copy_out(rd, buf, acc);
+	rend_staticrelease();
+	return rays;
+}
+
+unsigned ref_render_single(struct rdata *rd, unsigned bszx, unsigned bszy, unsigned frames,
+                           unsigned *buf, float *acc)
+{
+	rend_staticinit(bszx, bszy);
+	rend_clraccum(rd);
+	memset(rd->buf, 0, (size_t)rd->width * rd->height * sizeof(unsigned));
+	float xofs[8] = {0, 1, 2, 3, 0, 1, 2, 3}, yofs[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+	__m256 xo = _mm256_set_ps(3, 2, 1, 0, 3, 2, 1, 0);
+	__m256 yo = _mm256_set_ps(1, 1, 1, 1, 0, 0, 0, 0);
+	unsigned blkcntx = rd->width / bszx, blkcnt = blkcntx * (rd->height / bszy);
+	unsigned npk = bszx * bszy / 8, rays = 0;
+	for (unsigned f = 0; f < frames; f++) {
+		float invs = 1.0f / (rd->spp + rd->samples);
+		for (unsigned blk = 0; blk < blkcnt; blk++) {
+			struct rngstate8 rng;
+			rand8_init(&rng, ((int)blk + 13) * 1571, rd->samples * 23);
+			for (unsigned k = 0; k < npk; k++) {
+				unsigned px = (blk % blkcntx) * bszx + mortdecx(k) * 4;
+				unsigned py = (blk / blkcntx) * bszy + mortdecy(k) * 2;
+				__m256 v[6];
+				make_camray_pckt(&v[0], &v[1], &v[2], &v[3], &v[4], &v[5], px, py,
+				  xo, yo, &rd->cam, &rng);
+				struct vec3 col[8];
+				trace_rays(col, v[0], v[1], v[2], v[3], v[4], v[5], rd, &rays);
+				accum_pckt(rd->acc, rd->buf, col, px, py, xofs, yofs, rd->width, invs);
+			}
+		}
+		rd->samples += rd->spp;
+	}
+	copy_out(rd, buf, acc);
+	rend_staticrelease();
+	return rays;
 }

@@ -8,9 +8,13 @@ records
     reference's rend_prepstatic/rend_prepdynamic write, and
   * rays with the reference's intersect_tlas hits and intersect_any_tlas
     results (random, aimed and edge-case rays),
-into tests/golden/ref_<name>.npz.  The fixtures are data only.
+into tests/golden/ref_<name>.npz, and for two scenes three whole frames of
+the reference's rend_render (normal-tint shading, 64x32 blocks, its own
+camera rays) and of the same block walk traced with its single-ray
+trace_rays, into tests/golden/frames_<name>.npz.  The fixtures are data
+only.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [frames]
 """
 from __future__ import annotations
 
@@ -45,9 +49,40 @@ def regions(scene: Scene):
     return out
 
 
+#: name -> (config, detail, width, height) of the frame fixtures
+FRAME_SCENES = {"box": (1, 1.0, 128, 128), "terrain": (2, 0.2, 192, 128)}
+FRAMES = 3
+BLOCK = (64, 32)
+
+
+def frames():
+    for name, (cfg, det, w, h) in FRAME_SCENES.items():
+        s = Scene(config=cfg, detail=det, width=w, height=h)
+        r = RefScene(s)
+        buf_p, acc_p, rays_p = r.frames(s, FRAMES, block=BLOCK, threads=4)
+        buf_s, acc_s, rays_s = r.frames(s, FRAMES, block=BLOCK, single=True)
+        # rend_render's frame is stored as the pixels where it differs from
+        # the single-ray frame (the packet traversal's own hits)
+        diff = np.flatnonzero((acc_p != acc_s).any(axis=2) | (buf_p != buf_s))
+        np.savez_compressed(HERE / f"frames_{name}.npz", config=cfg, detail=det,
+                            width=w, height=h, frames=FRAMES, block=np.array(BLOCK),
+                            buf_single=buf_s, acc_single=acc_s, rays_single=rays_s,
+                            rays_packets=rays_p, packet_diff_px=diff,
+                            buf_packets_diff=buf_p.reshape(-1)[diff],
+                            acc_packets_diff=acc_p.reshape(-1, 3)[diff])
+        print(f"{name}: {w}x{h}, {FRAMES} frames, {rays_p} rays, "
+              f"{len(diff)} pixels where packets differ")
+        r.close()
+        s.close()
+
+
 def main():
     if reference() is None:
         subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True)
+    if sys.argv[1:] == ["frames"]:
+        frames()
+        return
+    frames()
     for name, (cfg, det) in SCENES.items():
         s = Scene(config=cfg, detail=det)
         assert s.hazards() == 0, f"{name}: reference build undefined (hazard {s.hazards()})"

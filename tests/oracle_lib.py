@@ -149,6 +149,14 @@ def reference(fast: bool = False) -> C.CDLL | None:
     L.ref_rsqrtps.argtypes = [C.c_float]
     L.ref_camrays.restype = None
     L.ref_camrays.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_void_p]
+    L.ref_frame_setup.restype = None
+    L.ref_frame_setup.argtypes = [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint]
+    L.ref_render.restype = C.c_uint
+    L.ref_render.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_int, C.c_void_p,
+                             C.c_void_p]
+    L.ref_render_single.restype = C.c_uint
+    L.ref_render_single.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_void_p,
+                                    C.c_void_p]
     _ref[path] = L
     return L
 
@@ -179,6 +187,25 @@ class RefScene:
         occ = np.zeros(len(rays), np.uint8)
         self.L.ref_occluded(C.c_void_p(self.rd), ptr(rays), ptr(occ), len(rays), threads)
         return occ
+
+    def frames(self, scene: Scene, frames: int, block=(64, 32), threads: int = 4,
+               single: bool = False):
+        """The reference's own rend_render frames (single=False) or the same
+        block walk with every packet traced by its single-ray trace_rays
+        (single=True) for the scene's camera, materials and resolution:
+        returns (ARGB buffer h x w, accumulator h x w x 3, rays)."""
+        v = scene.view()
+        self.L.ref_frame_setup(C.c_void_p(self.rd), C.c_void_p(scene.rd), scene.nmtls,
+                               scene.ntris)
+        buf = np.zeros(v.width * v.height, np.uint32)
+        acc = np.zeros(v.width * v.height * 3, np.float32)
+        if single:
+            n = self.L.ref_render_single(C.c_void_p(self.rd), block[0], block[1], frames,
+                                         ptr(buf), ptr(acc))
+        else:
+            n = self.L.ref_render(C.c_void_p(self.rd), block[0], block[1], frames, threads,
+                                  ptr(buf), ptr(acc))
+        return buf.reshape(v.height, v.width), acc.reshape(v.height, v.width, 3), n
 
     def close(self):
         if self.rd:

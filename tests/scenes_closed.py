@@ -81,3 +81,71 @@ def cube_coverage(img, albedo):
     the albedo as stored (float32)."""
     a = np.asarray(albedo, np.float32).astype(np.float64)
     return (1.0 - img) / (1.0 - a)
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: vertices (n x 3) on the sphere and triangles (m x 3),
+    each face split in four per level, new vertices projected outwards."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+         (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    for _ in range(subdiv):
+        mid, nf = {}, []
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[k] = len(v) - 1
+            return mid[k]
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(v), np.array(f, np.int64)
+
+
+def integrating_sphere(albedo, le, cap_y, subdiv=5, w=16, h=16, spp=1):
+    """Closed sphere of radius 1 seen from its centre: triangles whose
+    centroid lies above y = cap_y emit le (and reflect nothing), the rest
+    are Lambertian with the given albedo.  Returns (scene, emitter area
+    fraction of the mesh).  The camera looks down, away from the cap."""
+    v, f = icosphere(subdiv)
+    f = f[:, [0, 2, 1]]  # wound to face the inside (emitters emit on their front)
+    cen = v[f].mean(axis=1)
+    emit = cen[:, 1] > cap_y
+    e0, e1 = v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]
+    area = 0.5 * np.linalg.norm(np.cross(e0, e1), axis=1)
+    frac = float(area[emit].sum() / area.sum())
+    vv = np.ascontiguousarray(v, np.float32)
+    nn = np.ascontiguousarray(-v / np.linalg.norm(v, axis=1, keepdims=True), np.float32)
+    ii = np.ascontiguousarray(f, np.uint32)
+
+    def fill(L, rs):
+        mw = L.rs_material(rs, albedo, albedo, albedo, 0)
+        ml = L.rs_material(rs, le, le, le, 0x1000)
+        mt = np.ascontiguousarray(np.where(emit, ml, mw), np.uint32)
+        assert L.rs_mesh_begin(rs) == 0
+        assert L.rs_mesh_data(rs, vv.ctypes.data, len(vv), nn.ctypes.data, ii.ctypes.data,
+                              len(ii), mt.ctypes.data, 0) == 0
+        mesh = L.rs_mesh_end(rs)
+        assert L.rs_instance(rs, mesh, F3(0, 0, 0), Q4(0, 0, 0, 1), F3(1, 1, 1), 0) == 0
+        assert L.rs_camera(rs, F3(0, 0, 0), F3(0.01, -1, 0.02), 60.0, 1.0, 0.0) == 0
+        assert L.rs_background(rs, 0.0, 0.0, 0.0) == 0
+
+    return Scene.from_builder(fill, width=w, height=h, spp=spp), frac
+
+
+def integrating_sphere_radiance(albedo, le, frac, depth):
+    """Radiance of the sphere's wall estimated with `depth` path vertices,
+    each with next-event estimation (emitters seen by BSDF sampling add
+    nothing): on a sphere every surface element sees any region in
+    proportion to its area, so the wall's radiance is uniform and
+    L_D = a Le f sum_{k<D} (a (1 - f))^k."""
+    q = albedo * (1.0 - frac)
+    return albedo * le * frac * (1.0 - q ** depth) / (1.0 - q)

@@ -27,7 +27,8 @@ def sample_vmf_mixture(o, u):
     j = np.minimum((u[:, :1] >= cdf).sum(axis=1), G.LOBES - 1)
     n = np.arange(len(o))
     m, kk = mu[n, j], k[n, j]
-    w = 1 + np.log(u[:, 1] + (1 - u[:, 1]) * np.exp(-2 * kk)) / kk
+    v = 1 - u[:, 1]
+    w = 1 + np.log(v + (1 - v) * np.exp(-2 * kk)) / kk
     sw = np.sqrt(np.maximum(0, 1 - w * w))
     phi = 2 * np.pi * u[:, 2]
     sgn = np.where(m[:, 2] >= 0, 1.0, -1.0)
@@ -111,3 +112,58 @@ def test_loss_gradient_matches_finite_differences(bsdf):
         pm[i] -= h
         fd = (G.loss(pp, rec, gmin, ginv, bsdf) - G.loss(pm, rec, gmin, ginv, bsdf)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-5 + 2e-3 * abs(fd), (i, fd, g[i])
+
+
+def _host_sample(mix, u):
+    import ctypes as C
+
+    from nodebleed_amd import lib
+    out = np.zeros(3, np.float32)
+    m = np.ascontiguousarray(mix, np.float32)
+    rc = lib().rend_hip_guide_sample_host(C.c_void_p(m.ctypes.data), float(u[0]), float(u[1]),
+                                          float(u[2]), C.c_void_p(out.ctypes.data))
+    assert rc == 0
+    return out
+
+
+def _mix_record(pi, mu, k):
+    rec = np.zeros(5 * G.LOBES, np.float32)
+    rec[0::5], rec[1::5], rec[2::5], rec[3::5], rec[4::5] = pi, mu[:, 0], mu[:, 1], mu[:, 2], k
+    return rec
+
+
+def test_product_sampler_matches_restatement():
+    """The device sampling routine (compiled for the host) draws the same
+    directions as the numpy restatement for the same uniforms."""
+    rng = np.random.default_rng(5)
+    o = random_outputs(400, 11)
+    u = rng.random((400, 3))
+    want = sample_vmf_mixture(o, u)
+    pi, mu, k = G.mixture(o)
+    for i in range(len(o)):
+        got = _host_sample(_mix_record(pi[i], mu[i], k[i]), u[i])
+        assert np.all(np.isfinite(got))
+        assert np.abs(got - want[i]).max() < 2e-3, (i, got, want[i])
+
+
+@pytest.mark.parametrize("logk", [-2.0, 2.0, 7.0])
+def test_sampler_is_finite_at_the_ends_of_the_unit_interval(logk):
+    """u1 = 0 (a 24-bit uniform can be exactly 0) with a concentration whose
+    e^{-2k} underflows gives the lobe's mean direction, not log(0) = -inf;
+    the largest u1 gives the widest finite angle."""
+    k = float(np.exp(logk))
+    mu = np.zeros((G.LOBES, 3))
+    mu[:, 1] = 1.0
+    pi = np.full(G.LOBES, 1.0 / G.LOBES)
+    rec = _mix_record(pi, mu, np.full(G.LOBES, k))
+    for u1 in (0.0, 1.0 - 2.0 ** -24):
+        for u2 in (0.0, 0.3, 1.0 - 2.0 ** -24):
+            d = _host_sample(rec, (0.5, u1, u2))
+            assert np.all(np.isfinite(d)), (k, u1, u2, d)
+            assert abs(np.linalg.norm(d) - 1.0) < 1e-5
+            if u1 == 0.0:
+                assert np.allclose(d, (0.0, 1.0, 0.0), atol=1e-6)
+            else:
+                v = 2.0 ** -24
+                w = 1 + np.log(v + (1 - v) * np.exp(-2 * k)) / k
+                assert abs(d[1] - w) < 1e-4 * max(1.0, abs(w)), (k, d, w)

@@ -7,8 +7,18 @@ pinned:
   * next-event estimation: direct light on a floor from a square emitter
     equals albedo/pi times Lambert's polygon irradiance (which a dropped
     cosine, a wrong pdf conversion or a missing 1/pi would break);
+  * multi-bounce throughput, path depth and Russian roulette: inside a
+    closed diffuse sphere with an emissive cap the wall radiance has the
+    closed form a Le f sum_{k<D} (a (1 - f))^k for D path vertices
+    (integrating-sphere property); a dropped albedo factor, a missing 1/q
+    after a roulette survival or a depth off by one each miss it by far more
+    than the tolerance;
+  * Russian roulette is unbiased on a scene without a closed form (the box
+    of the path-tracing benchmark): roulette from the first bounce and no
+    roulette agree within the Monte Carlo error;
   * Monte Carlo convergence of the oracle's images themselves."""
 import numpy as np
+import pytest
 
 import scenes_closed as S
 from oracle_lib import or_pt64
@@ -53,3 +63,55 @@ def test_polygon_irradiance_formula():
     d, h = 1e-3, 1.0
     sq = [(-d / 2, h, -d / 2), (d / 2, h, -d / 2), (d / 2, h, d / 2), (-d / 2, h, d / 2)]
     assert abs(S.polygon_irradiance((0, 0, 0), (0, 1, 0), sq, 1.0) / (d * d / h / h) - 1) < 1e-5
+
+
+SPHERE = dict(albedo=0.5, le=1.0, cap_y=0.8)
+
+
+@pytest.mark.parametrize("depth,rr", [(1, 0), (2, 0), (4, 0), (4, 1), (8, 3), (8, 1)])
+def test_oracle_integrating_sphere_closed_form(depth, rr):
+    s, frac = S.integrating_sphere(SPHERE["albedo"], SPHERE["le"], SPHERE["cap_y"], subdiv=5)
+    img = or_pt64(s, 2048, maxdepth=depth, rr_depth=rr, seed=3)
+    s.close()
+    want = S.integrating_sphere_radiance(SPHERE["albedo"], SPHERE["le"], frac, depth)
+    # 256 pixels x 2048 paths: the mean's standard error is < 0.1 % (with
+    # roulette from the first bounce); the 20,480-facet mesh is within
+    # 3e-4 of the smooth sphere's closed form
+    got = img.mean()
+    assert abs(got / want - 1) < 3e-3, (depth, rr, got, want)
+    # the same tolerance separates every depth from the next one
+    nxt = S.integrating_sphere_radiance(SPHERE["albedo"], SPHERE["le"], frac, depth + 1)
+    assert nxt / want - 1 > 3 * 3e-3 or depth >= 8
+    # and the wall is uniform: every pixel is within its own noise of it
+    assert np.abs(img / want - 1).max() < 0.2
+
+
+def test_closed_form_detects_planted_mistakes():
+    """What the closed form would read for an estimator that drops the
+    albedo from the throughput, or forgets the 1/q after a roulette
+    survival (q = the surviving throughput's maximum, below 0.95), at
+    depth 4: both far outside the tolerance above."""
+    a, le, f = SPHERE["albedo"], SPHERE["le"], 0.1
+    good = S.integrating_sphere_radiance(a, le, f, 4)
+    no_albedo = a * le * f * sum((1 - f) ** k for k in range(4))
+    # roulette from the first bounce without 1/q: vertex k survives with
+    # prod_{j<=k} min(a^j, 0.95) and is not re-weighted
+    no_inv_q = sum(a * le * f * (a * (1 - f)) ** k * np.prod([min(a ** j, 0.95)
+                   for j in range(1, k + 1)]) for k in range(4))
+    assert abs(no_albedo / good - 1) > 0.5
+    assert abs(no_inv_q / good - 1) > 0.1
+
+
+def test_oracle_russian_roulette_is_unbiased_on_the_box():
+    from nodebleed_amd import Scene
+    s = Scene(config=1, detail=0.25, width=16, height=16)
+    off = or_pt64(s, 1024, maxdepth=6, rr_depth=0, seed=11)
+    on = or_pt64(s, 1024, maxdepth=6, rr_depth=1, seed=12)
+    s.close()
+    # whole-image means: 262,144 paths each, relative standard error of the
+    # roulette image ~0.3 %
+    assert abs(on.mean() / off.mean() - 1) < 0.015, (on.mean(), off.mean())
+    # and per 4x4 block
+    b_on = on.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
+    b_off = off.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
+    assert np.abs(b_on - b_off).max() < 0.1 * b_off.max()

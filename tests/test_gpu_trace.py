@@ -208,7 +208,7 @@ def test_render_frame_protocol():
     for t in th:
         t.join()
     v = s.view()
-    assert L.rend_hip_lasterror() == 0 or True
+    # rays are added only by a frame that rendered without error
     assert v.rays == 256 * 256, (v.rays, L.rend_hip_errstr())
     assert v.blknum >= (256 // 64) * (256 // 32)
     assert (buf >> 24 == 0xFF).all()
