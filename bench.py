@@ -42,23 +42,45 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(st: TraceStats, nrays: int) -> int:
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD for
+# 4 cycles; engine clock 2.4 GHz (peak).
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
+HOST_SHARE = 16
+PMC_PROFILE = "profiles/trace_terrain_diffuse_pmc.md"
+
+
+def visit_bytes(st: TraceStats, nrays: int) -> int:
+    """Bytes the traversal's visits read plus ray and hit records: what the
+    caches deliver to the lanes (most of it from L1/L2/MALL)."""
     return nrays * S_RAY + st.nodes * S_NODE + st.leaves * S_LEAF + st.insts * S_INST
 
 
-def measured_traffic():
-    """HBM bytes per diffuse-ray launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the ratio the
-    same counters show for a plain copy kernel of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "trace_terrain_diffuse_pmc.md")
+def compulsory_bytes(scene: Scene, nrays: int) -> int:
+    """The least a launch must move across HBM when nothing stays cached
+    between launches: every ray record read and every hit written once, and
+    the scene's device tree (BLAS + TLAS arena) and instance records read
+    once."""
+    L = lib()
+    arena = int(L.rend_hip_arena_bytes(scene.rd))
+    return nrays * S_RAY + max(arena, 0) + 64 * int(scene.view().instcnt)
+
+
+def profile_counters():
+    """Per-launch figures of the committed rocprofv3 --pmc summary of this
+    workload: HBM bytes (FETCH_SIZE + WRITE_SIZE scaled by a copy kernel's
+    reading) and SQ_INSTS_VALU; None where absent."""
+    out = {"hbm_bytes": None, "valu_insts": None}
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, PMC_PROFILE)) as f:
             for line in f:
-                if line.strip().startswith("hbm_bytes_per_launch:"):
-                    return int(line.split(":", 1)[1].strip().split()[0])
-    except (OSError, ValueError):
+                t = line.strip()
+                if t.startswith("hbm_bytes_per_launch:"):
+                    out["hbm_bytes"] = int(t.split(":", 1)[1].strip().split()[0])
+                elif t.startswith("| SQ_INSTS_VALU |"):
+                    out["valu_insts"] = int(t.split("|")[2].strip().replace(",", ""))
+    except (OSError, ValueError, IndexError):
         pass
-    return None
+    return out
 
 
 def timed_launches(L, scene, d_rays, d_hits, n, k):
@@ -81,8 +103,11 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
     cores, over a bounded prefix of the same diffuse rays."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(int(threads), 16))
+    online = os.sysconf("SC_NPROCESSORS_ONLN") if hasattr(os, "sysconf") else os.cpu_count()
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else online
+    # the reference runs one thread per online core (main.c:386); a one-GPU
+    # share of the GPU host is 16 threads, whatever the host reports
+    threads = max(1, min(int(allowed), HOST_SHARE))
     ref = O.reference(fast=True)
     if ref is not None:
         rs = O.RefScene(scene, fast=True)
@@ -118,7 +143,9 @@ def cpu_baseline(scene: Scene, rays: np.ndarray, seconds: float) -> dict:
     except OSError:
         pass
     return {"value": round(n / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads,
-            "kind": kind,
+            "kind": kind, "online_cores": int(online), "allowed_cores": int(allowed),
+            "thread_policy": f"one per allowed core, capped at the {HOST_SHARE}-thread "
+                             "host share of one GPU",
             "sample": f"{what}, {n} rays ({passes} pass(es) over the first "
                       f"{n // passes} of {len(rays)} diffuse rays), {threads} threads "
                       f"on {cpu}, {dt:.1f} s"}
@@ -519,8 +546,10 @@ def main():
     ms_step = ms_d / a.steps
     mrays = n / (ms_step * 1e-3) / 1e6
     mrays_p = n / (ms_p / a.steps * 1e-3) / 1e6
-    bytes_d = algorithmic_bytes(st_d, n)
-    achieved = bytes_d / (ms_step * 1e-3) / 1e9
+    bytes_alg = compulsory_bytes(scene, n)
+    achieved = bytes_alg / (ms_step * 1e-3) / 1e9
+    bytes_vis = visit_bytes(st_d, n)
+    prof = profile_counters()
 
     cpu = None
     if not a.no_cpu:
@@ -559,10 +588,27 @@ def main():
                                "insts": round(st_d.insts / n, 3)},
             "device": f"{name.value.decode()} ({cus.value} CUs)",
         },
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(),
-                     "algorithmic_bytes_per_launch": int(bytes_d)},
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": prof["hbm_bytes"],
+            "algorithmic_bytes_per_launch": int(bytes_alg),
+            "algorithmic_bytes": "rays x 48 B (32 B ray in, 16 B hit out) + device "
+                                 "arena (BLAS + TLAS) + 64 B per instance, once per launch",
+            "traffic_source": f"{PMC_PROFILE} (FETCH_SIZE + WRITE_SIZE per launch, "
+                              "copy-kernel scaled)",
+            "hbm_frac_measured": (round(prof["hbm_bytes"] / (ms_step * 1e-3) / 1e9
+                                        / PEAK_HBM_GBS, 4) if prof["hbm_bytes"] else None),
+            "visit_bytes_per_launch": int(bytes_vis),
+            "visit_bytes_rate_GBs": round(bytes_vis / (ms_step * 1e-3) / 1e9, 1),
+            "visit_bytes": "node 228 B, leaf 160 B, instance 56 B per visit of the "
+                           "reference algorithm + ray records: served by L1/L2/MALL",
+            "valu_issue_frac": (round(prof["valu_insts"] * VALU_CYCLES / SIMDS / CLOCK_HZ
+                                      / (ms_step * 1e-3), 4) if prof["valu_insts"] else None),
+            "valu_source": f"{PMC_PROFILE} (SQ_INSTS_VALU per launch x 4 cycles / "
+                           "(1024 SIMDs x 2.4 GHz x launch time))",
+            "limiter": "VALU issue and memory latency of divergent traversal; HBM "
+                       "bandwidth is not the binding ceiling"},
         "cpu_baseline": cpu,
     }
     scene.close()

@@ -56,6 +56,7 @@ struct pt64 {
 	struct dlight *lights;
 	unsigned nlights;
 	double (*fwd)[12];   /* world matrix (inverse of globinv) per instance */
+	int use_tlas;        /* prune instances with the host TLAS */
 	double *img;
 	unsigned next_row;
 };
@@ -193,6 +194,33 @@ static int blas64(const struct rdata *rd, unsigned triofs, const struct dray *r,
 	return hit;
 }
 
+/* One instance: the world ray through globinv into the BLAS. */
+static int inst64(const struct rdata *rd, unsigned i, const struct dray *wr,
+                  double *t, double *u, double *v, unsigned *tri, int any)
+{
+	const struct aabb *b = &rd->aabbs[i];
+	double mn[3] = {b->min.x, b->min.y, b->min.z};
+	double mx[3] = {b->max.x, b->max.y, b->max.z};
+	double tn;
+	if (!box_hit(wr, mn, mx, *t, &tn))
+		return 0;
+	const float *g = rd->insts[i].globinv;
+	double o[3], d[3];
+	for (int k = 0; k < 3; k++) {
+		o[k] = g[4 * k] * wr->o[0] + g[4 * k + 1] * wr->o[1] +
+		  g[4 * k + 2] * wr->o[2] + g[4 * k + 3];
+		d[k] = g[4 * k] * wr->d[0] + g[4 * k + 1] * wr->d[1] +
+		  g[4 * k + 2] * wr->d[2];
+	}
+	struct dray br;
+	mk_ray(&br, o, d);
+	return blas64(rd, rd->insts[i].triofs, &br, t, u, v, tri, any);
+}
+
+/* Closest or any hit over all instances.  The TLAS (rd->bnodes at tlasofs,
+ * children8: byte offsets, leaves NODE_LEAF | instance id) only prunes:
+ * its boxes enclose the instance boxes, each tested conservatively, so the
+ * result is that of testing every instance. */
 static int scene64(const struct pt64 *P, const struct dray *wr, double tmax,
                    double *t, double *u, double *v, unsigned *inst,
                    unsigned *tri, int any)
@@ -200,6 +228,37 @@ static int scene64(const struct pt64 *P, const struct dray *wr, double tmax,
 	const struct rdata *rd = P->rd;
 	*t = tmax;
 	int hit = 0;
+	if (P->use_tlas) {
+		const unsigned char *base = (const unsigned char *)&rd->bnodes[rd->tlasofs];
+		unsigned stack[512];
+		unsigned sp = 0;
+		stack[sp++] = 0;
+		while (sp) {
+			unsigned ofs = stack[--sp];
+			if (ofs & LEAF) {
+				unsigned i = ofs & ~LEAF;
+				if (i < rd->instcnt && inst64(rd, i, wr, t, u, v, tri, any)) {
+					*inst = i;
+					hit = 1;
+					if (any)
+						return 1;
+				}
+				continue;
+			}
+			const struct bnode8 *n = (const struct bnode8 *)(base + ofs);
+			for (unsigned c8 = 0; c8 < 8; c8++) {
+				unsigned c = U(n->children8, c8);
+				if (c == 0)
+					continue;
+				double mn[3] = {F(n->minx8, c8), F(n->miny8, c8), F(n->minz8, c8)};
+				double mx[3] = {F(n->maxx8, c8), F(n->maxy8, c8), F(n->maxz8, c8)};
+				double tn;
+				if (box_hit(wr, mn, mx, *t, &tn) && sp < 512)
+					stack[sp++] = c;
+			}
+		}
+		return hit;
+	}
 	for (unsigned i = 0; i < rd->instcnt; i++) {
 		const struct aabb *b = &rd->aabbs[i];
 		double mn[3] = {b->min.x, b->min.y, b->min.z};
@@ -451,6 +510,67 @@ static int build_lights(struct pt64 *P)
 	return 0;
 }
 
+/* The host TLAS may prune only if it is sound: every instance reachable
+ * exactly once, and every child box containing the boxes below it (the
+ * builder's degenerate splits can leave stale boxes).  Returns 1 if so. */
+static int tlas_check(const struct rdata *rd, unsigned ofs, const double *bmin,
+                      const double *bmax, unsigned char *seen, int depth)
+{
+	if (depth > 64)
+		return 0;
+	if (ofs & LEAF) {
+		unsigned i = ofs & ~LEAF;
+		if (i >= rd->instcnt || seen[i])
+			return 0;
+		seen[i] = 1;
+		const struct aabb *b = &rd->aabbs[i];
+		double mn[3] = {b->min.x, b->min.y, b->min.z}, mx[3] = {b->max.x, b->max.y, b->max.z};
+		for (int k = 0; k < 3; k++)
+			if (bmin && (mn[k] < bmin[k] || mx[k] > bmax[k]))
+				return 0;
+		return 1;
+	}
+	if (ofs % 32 || ofs >= 2u * (rd->instcnt + 1) * sizeof(struct bnode8))
+		return 0;
+	const struct bnode8 *n = (const struct bnode8 *)((const unsigned char *)
+	  &rd->bnodes[rd->tlasofs] + ofs);
+	for (unsigned c8 = 0; c8 < 8; c8++) {
+		unsigned c = U(n->children8, c8);
+		if (c == 0)
+			continue;
+		double mn[3] = {F(n->minx8, c8), F(n->miny8, c8), F(n->minz8, c8)};
+		double mx[3] = {F(n->maxx8, c8), F(n->maxy8, c8), F(n->maxz8, c8)};
+		for (int k = 0; k < 3; k++)
+			if (bmin && (mn[k] < bmin[k] || mx[k] > bmax[k]))
+				return 0;
+		if (!tlas_check(rd, c, mn, mx, seen, depth + 1))
+			return 0;
+	}
+	return 1;
+}
+
+static int tlas_sound(const struct rdata *rd)
+{
+	if (rd->instcnt < 2)
+		return 0;
+	unsigned char *seen = calloc(rd->instcnt, 1);
+	if (!seen)
+		return 0;
+	int ok = tlas_check(rd, 0, NULL, NULL, seen, 0);
+	for (unsigned i = 0; ok && i < rd->instcnt; i++)
+		ok = seen[i];
+	free(seen);
+	return ok;
+}
+
+static int g_tlas_mode = 1;
+
+/* 0: test every instance; 1: prune with the host TLAS where it is sound. */
+void or_pt64_tlas(int mode)
+{
+	g_tlas_mode = mode;
+}
+
 /* Renders rd's camera (cam: eye, ri, up, fwd, rw, rh, aspect, fdist,
  * fovfdist as 17 doubles) at w x h with spp samples per pixel into img
  * (3 doubles per pixel, averaged). */
@@ -480,6 +600,7 @@ int or_pt64(const struct rdata *rd, const double *cam, unsigned w, unsigned h,
 	P.bg[1] = rd->bgcol.y;
 	P.bg[2] = rd->bgcol.z;
 	P.img = img;
+	P.use_tlas = g_tlas_mode && rd->instcnt > 16 && tlas_sound(rd);
 	if (build_lights(&P))
 		return -1;
 	if (threads < 1)

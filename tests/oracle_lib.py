@@ -38,6 +38,8 @@ def oracle() -> C.CDLL:
         L.or_pt64.restype = C.c_int
         L.or_pt64.argtypes = [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, C.c_uint,
                               C.c_uint, C.c_uint, C.c_ulonglong, C.c_int, C.c_void_p]
+        L.or_pt64_tlas.restype = None
+        L.or_pt64_tlas.argtypes = [C.c_int]
         L.or_rcp.restype = C.c_float
         L.or_rcp.argtypes = [C.c_float]
         for f in ("or_xmax", "or_xmin"):

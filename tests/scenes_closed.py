@@ -52,6 +52,50 @@ def lit_floor_scene(w=4, h=4, spp=1):
     return Scene.from_builder(fill, width=w, height=h, spp=spp)
 
 
+def lit_floor_two_instances(w=4, h=4, spp=1):
+    """The lit floor with the floor and the emitter as two instances of two
+    meshes (instance 1 = the emitter, centred at the origin of its mesh), so
+    the emitter can be moved by its instance transform."""
+    def fill(L, rs):
+        mf = L.rs_material(rs, FLOOR_ALBEDO, FLOOR_ALBEDO, FLOOR_ALBEDO, 0)
+        ml = L.rs_material(rs, LIGHT_LE, LIGHT_LE, LIGHT_LE, 0x1000)
+        assert L.rs_mesh_begin(rs) == 0
+        assert L.rs_quad(rs, F3(-5, 0, -5), F3(5, 0, -5), F3(5, 0, 5), F3(-5, 0, 5), mf) == 0
+        floor = L.rs_mesh_end(rs)
+        assert L.rs_mesh_begin(rs) == 0
+        a, y = LIGHT_HALF, LIGHT_Y
+        assert L.rs_quad(rs, F3(-a, y, -a), F3(a, y, -a), F3(a, y, a), F3(-a, y, a), ml) == 0
+        light = L.rs_mesh_end(rs)
+        assert L.rs_instance(rs, floor, F3(0, 0, 0), Q4(0, 0, 0, 1), F3(1, 1, 1), 0) == 0
+        assert L.rs_instance(rs, light, F3(0, 0, 0), Q4(0, 0, 0, 1), F3(1, 1, 1), 0) == 1
+        assert L.rs_camera(rs, F3(2.0, 0.5, 0.0), F3(*TARGET), 1.0, 1.0, 0.0) == 0
+        assert L.rs_background(rs, 0.0, 0.0, 0.0) == 0
+
+    return Scene.from_builder(fill, width=w, height=h, spp=spp)
+
+
+def move_instance(scene, i, shift):
+    """Translates instance i of a scene whose instances have no rotation or
+    scale, as the application's upd_rinsts does (main.c:117-157): new
+    globinv and world box in the host rdata, then rend_prepdynamic."""
+    ins, ab = scene.insts(), scene.aabbs()
+    g = ins["globinv"][i].reshape(3, 4)
+    g[:, 3] -= np.asarray(shift, np.float32)
+    for k, key in enumerate(("min", "max")):
+        ab[key][i] += np.asarray(shift, np.float32)
+    ab["min"][i] -= 1e-5
+    ab["max"][i] += 1e-5
+    scene.L.rend_prepdynamic(scene.rd)
+
+
+def lit_floor_radiance_at(light_centre):
+    a = LIGHT_HALF
+    cx, cy, cz = light_centre
+    poly = [(cx - a, cy, cz - a), (cx + a, cy, cz - a), (cx + a, cy, cz + a),
+            (cx - a, cy, cz + a)]
+    return FLOOR_ALBEDO / np.pi * polygon_irradiance(TARGET, (0, 1, 0), poly, LIGHT_LE)
+
+
 def polygon_irradiance(x, n, poly, Le):
     """Irradiance at x (normal n) from a Lambertian polygon of radiance Le
     (Lambert's formula: E = Le/2 * |sum_i theta_i n . g_i|, theta_i the angle
@@ -138,7 +182,9 @@ def integrating_sphere(albedo, le, cap_y, subdiv=5, w=16, h=16, spp=1):
         assert L.rs_camera(rs, F3(0, 0, 0), F3(0.01, -1, 0.02), 60.0, 1.0, 0.0) == 0
         assert L.rs_background(rs, 0.0, 0.0, 0.0) == 0
 
-    return Scene.from_builder(fill, width=w, height=h, spp=spp), frac
+    s = Scene.from_builder(fill, width=w, height=h, spp=spp)
+    s.emitters = int(emit.sum())
+    return s, frac
 
 
 def integrating_sphere_radiance(albedo, le, frac, depth):

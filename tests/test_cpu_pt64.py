@@ -115,3 +115,54 @@ def test_oracle_russian_roulette_is_unbiased_on_the_box():
     b_on = on.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
     b_off = off.reshape(4, 4, 4, 4, 3).mean(axis=(1, 3))
     assert np.abs(b_on - b_off).max() < 0.1 * b_off.max()
+
+
+def test_oracle_tlas_pruning_changes_nothing():
+    """The oracle prunes instances with the host TLAS when that tree is
+    sound; images are identical to testing every instance."""
+    from nodebleed_amd import Scene
+    from oracle_lib import oracle
+    s = Scene(config=3, detail=0.03, width=16, height=12)
+    try:
+        a = or_pt64(s, 16, maxdepth=8, rr_depth=3, seed=5)
+        oracle().or_pt64_tlas(0)
+        b = or_pt64(s, 16, maxdepth=8, rr_depth=3, seed=5)
+    finally:
+        oracle().or_pt64_tlas(1)
+        s.close()
+    assert np.array_equal(a, b) and a.mean() > 0
+
+
+PT_FIXTURES = {"box_full": 16, "instances_full": 64, "occluded_full": 256}
+
+
+@pytest.mark.parametrize("name", sorted(PT_FIXTURES))
+def test_fp64_fixtures_follow_the_oracle(name):
+    """Each committed FP64 reference image (tests/golden/make_pt_golden.py)
+    is what the oracle renders now: a low-sample render of the same case
+    agrees with it within that render's own noise (block relMSE and image
+    mean), so a change of the estimator shows up here first."""
+    from pathlib import Path
+
+    from nodebleed_amd import Scene
+    from oracle_lib import rel_mse
+    f = np.load(Path(__file__).resolve().parent / "golden" / f"pt64_{name}.npz")
+    w, h, k = int(f["width"]), int(f["height"]), int(f["block"])
+    s = Scene(config=int(f["config"]), detail=float(f["detail"]), width=w, height=h)
+    try:
+        spp = PT_FIXTURES[name]
+        img = or_pt64(s, spp, maxdepth=int(f["maxdepth"]), rr_depth=int(f["rr_depth"]),
+                      seed=101)
+    finally:
+        s.close()
+    b = img.reshape(h // k, k, w // k, k, 3).mean(axis=(1, 3))
+    ref = f["blocks"].astype(np.float64)
+    # noise of the low-sample render alone: per-block sample variance from
+    # the two half-images would need the pixels; bound it by the measured
+    # relMSE levels of these cases (box 4e-4, instances 3.4e-3, occluded
+    # 3.4e-2 at these sample counts) with a factor 4 of margin
+    bound = {"box_full": 2e-3, "instances_full": 1.5e-2, "occluded_full": 0.15}[name]
+    err = rel_mse(b, ref)
+    assert err < bound, (name, err)
+    assert abs(b.mean() / ref.mean() - 1) < {"box_full": 0.01, "instances_full": 0.03,
+                                              "occluded_full": 0.1}[name]

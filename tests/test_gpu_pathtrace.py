@@ -8,6 +8,7 @@ at sample counts where the noise part of relMSE is ~1e-4, so a relMSE above
 """
 import ctypes as C
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -182,3 +183,85 @@ def test_autotune_picks_valid_shape(terrain):
     check(L.rend_hip_set_trace_variant(0))
     check(L.rend_hip_set_tuning(0, 48))
     check(L.rend_hip_pt_config(4, 3, 0))
+
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+#: fixture -> (spp per frame, frames) of the GPU render: at least the
+#: oracle's sample count, so the GPU image's noise is at most the fixture's
+GPU_SAMPLES = {"box_full": (256, 16), "instances_full": (1024, 16),
+               "occluded_full": (1024, 64)}
+
+
+@pytest.mark.parametrize("name", sorted(GPU_SAMPLES))
+def test_full_size_scene_matches_fp64_fixture(name):
+    """Each path-tracing benchmark scene at its full geometric size against
+    the FP64 oracle's image (tests/golden/make_pt_golden.py: the sphere in
+    the box at 256x256, 2x2-pixel blocks; the 4,096 instances at 64x36 and
+    depth 8; the occluded-lights terrain at 96x54 and depth 6 with the
+    oracle at 65,536 spp).  The fixtures' own noise is below 1.5e-4 of
+    relMSE, so relMSE <= 1e-3 bounds the bias."""
+    f = np.load(GOLDEN / f"pt64_{name}.npz")
+    spp, frames = GPU_SAMPLES[name]
+    w, h, k = int(f["width"]), int(f["height"]), int(f["block"])
+    say(f"{name}: building config {int(f['config'])} at {w}x{h}")
+    s = Scene(config=int(f["config"]), detail=float(f["detail"]), width=w, height=h, spp=spp)
+    try:
+        say(f"{name}: GPU {frames} frames x {spp} spp, depth {int(f['maxdepth'])}")
+        img, rays = s.render(frames=frames, maxdepth=int(f["maxdepth"]),
+                             rr_depth=int(f["rr_depth"]))
+    finally:
+        s.close()
+    assert np.isfinite(img).all() and (img >= 0).all()
+    err = rel_mse(blocks(img, k), f["blocks"])
+    say(f"{name}: relMSE({k}x{k} blocks) {err:.2e} against the oracle at "
+        f"{int(f['spp'])} spp, {rays} rays")
+    assert err <= 1e-3, err
+
+
+@pytest.mark.parametrize("cap_y,depth,rr", [(0.8, 4, 0), (0.8, 8, 1), (0.995, 4, 0),
+                                            (0.995, 6, 2)])
+def test_integrating_sphere_closed_form(cap_y, depth, rr):
+    """Multi-bounce throughput, path depth and Russian roulette against a
+    closed form: inside a closed diffuse sphere with an emissive cap the wall
+    radiance with D path vertices is a Le f sum_{k<D} (a (1 - f))^k.  The
+    cap is 2,044 emitter triangles (the area x luminance CDF branch of the
+    light choice) or 46 (the per-receiver importance branch).  IEEE
+    reciprocal: with the reference's 12-bit RCPSS triangle test a fraction
+    of rays slips between adjacent triangles and leaves the sphere."""
+    from nodebleed_amd import REND_RCP_HOST_TABLE, REND_RCP_IEEE, check, lib
+    s, frac = S.integrating_sphere(0.5, 1.0, cap_y, subdiv=5, w=16, h=16, spp=1024)
+    assert (s.emitters > 64) == (cap_y < 0.9)
+    say(f"sphere cap {cap_y} ({s.emitters} emitters), depth {depth}, rr {rr}: 4 x 1024 spp")
+    check(lib().rend_hip_set_rcp(REND_RCP_IEEE))
+    try:
+        img, _ = s.render(frames=4, maxdepth=depth, rr_depth=rr)
+    finally:
+        lib().rend_hip_set_rcp(REND_RCP_HOST_TABLE)
+        s.close()
+    want = S.integrating_sphere_radiance(0.5, 1.0, frac, depth)
+    say(f"sphere: mean {img.mean():.6g}, closed form {want:.6g}")
+    assert abs(img.mean() / want - 1) < 3e-3
+    assert np.abs(img / want - 1).max() < 0.1
+
+
+def test_moved_emitter_lights_from_its_new_place():
+    """Next-event estimation follows an emissive instance that
+    rend_prepdynamic moved between frames: the floor radiance matches the
+    polygon-irradiance closed form before and after the move."""
+    s = S.lit_floor_two_instances(w=4, h=4, spp=4096)
+    try:
+        say("moved emitter: before")
+        img0, _ = s.render(frames=8, maxdepth=1, rr_depth=0)
+        S.move_instance(s, 1, (0.4, 0.0, 0.0))
+        say("moved emitter: after moving the light by 0.4")
+        img1, _ = s.render(frames=8, maxdepth=1, rr_depth=0)
+    finally:
+        s.close()
+    want0 = S.lit_floor_radiance_at((0.0, S.LIGHT_Y, 0.0))
+    want1 = S.lit_floor_radiance_at((0.4, S.LIGHT_Y, 0.0))
+    assert want1 / want0 > 1.2
+    got0, got1 = img0[2, 1:4].mean(), img1[2, 1:4].mean()
+    say(f"moved emitter: {got0:.5g} (closed form {want0:.5g}) -> {got1:.5g} ({want1:.5g})")
+    assert abs(got0 / want0 - 1) < 0.01
+    assert abs(got1 / want1 - 1) < 0.01

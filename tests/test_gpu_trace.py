@@ -145,19 +145,21 @@ def test_ieee_reciprocal_mode(scenes):
         L.rend_hip_set_rcp(REND_RCP_HOST_TABLE)
 
 
-def test_full_terrain_gpu_rays():
-    """The benchmark workload at full size: GPU-generated primary and diffuse
-    rays on the 1M-triangle terrain; a 40k-ray sample of each buffer must be
-    bit-exact with the oracle, and every primary ray must hit."""
+@pytest.mark.parametrize("config", [2, 3])
+def test_full_scene_gpu_rays(config):
+    """The benchmark scenes at full size: GPU-generated 1920x1080 primary and
+    diffuse rays on the 1M-triangle terrain (config 2) and on the 4,096
+    instances of 16 meshes (config 3, 2.48 M unique triangles); a 40k-ray
+    sample of each buffer must be bit-exact with the oracle."""
     L = lib()
-    say("full terrain: building 1M-triangle scene")
-    s = Scene(config=2, detail=1.0, width=1920, height=1080)
+    say(f"full config {config}: building the scene")
+    s = Scene(config=config, detail=1.0, width=1920, height=1080)
     n = 1920 * 1080
     d = [L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize),
          L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize)]
     try:
         st = L.rend_hip_stream()
-        say("full terrain: camera rays, trace, diffuse rays, trace")
+        say(f"full config {config}: camera rays, trace, diffuse rays, trace")
         check(L.rend_hip_gen_camrays(s.rd, d[0], 1920, 1080, 1, 1, st))
         check(L.rend_hip_trace(s.rd, d[0], d[1], n, st, None))
         check(L.rend_hip_gen_diffuse(s.rd, d[0], d[1], d[2], n, 1, st))
@@ -169,13 +171,19 @@ def test_full_terrain_gpu_rays():
             check(L.rend_hip_memcpy_dtoh(a.ctypes.data, p, n * dt.itemsize))
             bufs.append(a)
         prim, phit, diff, dhit = bufs
-        # the camera frames only terrain; a few rays slip through shared
-        # edges (the Moeller-Trumbore test is not watertight), as on the CPU
-        assert (phit["id"] != 0xFFFFFFFF).mean() > 0.999
+        hitfrac = (phit["id"] != 0xFFFFFFFF).mean()
+        say(f"full config {config}: primary hit fraction {hitfrac:.4f}")
+        if config == 2:
+            # the camera frames only terrain; a few rays slip through shared
+            # edges (the Moeller-Trumbore test is not watertight), as on the CPU
+            assert hitfrac > 0.999
+        else:
+            assert hitfrac > 0.2
         rng = np.random.default_rng(5)
         for name, rays, hits in (("primary", prim, phit), ("diffuse", diff, dhit)):
-            idx = np.sort(rng.choice(n, 40_000, replace=False))
-            say(f"full terrain: oracle on {name} sample")
+            live = np.flatnonzero(rays["tmax"] > 0)
+            idx = np.sort(rng.choice(live, 40_000, replace=False))
+            say(f"full config {config}: oracle on {name} sample")
             want, _ = or_trace(s, rays[idx])
             assert mismatches(hits[idx], want) == {"id": 0, "t": 0, "u": 0, "v": 0}, name
         # size-independent properties over the whole buffer
@@ -183,7 +191,9 @@ def test_full_terrain_gpu_rays():
         ok = dhit["id"] != 0xFFFFFFFF
         assert (dhit["t"][ok] > 0).all() and (dhit["t"][ok] < diff["tmax"][ok]).all()
         assert (dhit["u"][ok] >= 0).all() and (dhit["v"][ok] >= 0).all()
-        assert ((dhit["id"][ok] & 0xFFF) == 0).all() and ((dhit["id"][ok] >> 12) < s.ntris).all()
+        ins = s.insts()
+        inst, tri = dhit["id"][ok] & 0xFFF, dhit["id"][ok] >> 12
+        assert (inst < len(ins)).all() and (tri < ins["tricnt"][inst]).all()
     finally:
         for p in d:
             L.rend_hip_free(p)
