@@ -107,7 +107,7 @@ int              rl_fail(int code, const char *fmt, ...);
 int              rl_hipfail(int herr, const char *what);
 int              rl_device(void);
 struct rl_scene *rl_scene(const struct rdata *rd);
-int              rl_trace_grid(int closest, int count, unsigned *grid);
+int              rl_trace_grid(int closest, int count, int rcp_ieee, unsigned *grid);
 int              rl_trace_async(struct rl_scene *s, const void *d_rays,
                                 void *d_hits, void *d_occ, unsigned n,
                                 int closest, int count, hipStream_t st);

@@ -263,6 +263,14 @@ static int pt_lights(struct rl_scene *s)
 	return 0;
 }
 
+/* The path tracer's traversal divides by the determinant exactly (IEEE
+ * 1/det), whatever rend_hip_set_rcp selects for the reference-exact entry
+ * points: a hit position p = o + t d feeds the next segment's origin, and
+ * with the host's 12-bit RCPSS reciprocal t carries a relative error of up
+ * to 3.7e-4, which at the instanced scene's distances (~100 units) puts the
+ * origin up to 0.04 below the surface, 40 times the 1e-3 spawn offset. */
+#define PT_RCP_IEEE 1
+
 static unsigned shade_grid(void)
 {
 	return (unsigned)G.cus * P.shade_mult;
@@ -284,7 +292,8 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 	size_t cap = t->batch;
 	int r;
 	unsigned cgrid, agrid;
-	if ((r = rl_trace_grid(1, 0, &cgrid)) || (r = rl_trace_grid(0, 0, &agrid)))
+	if ((r = rl_trace_grid(1, 0, PT_RCP_IEEE, &cgrid)) ||
+	    (r = rl_trace_grid(0, 0, PT_RCP_IEEE, &agrid)))
 		return r;
 	if ((r = rk_fill_f32((float *)&t->counts[4], 2, 0.0f, st)))
 		return rl_hipfail(r, "ray counter");
@@ -306,7 +315,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 		for (unsigned d = 0; d < a->maxdepth; d++) {
 			unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->rays[cur], t->hits,
-			  NULL, n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 1, 0, G.rcp_ieee, G.variant, cgrid,
+			  NULL, n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 1, 0, PT_RCP_IEEE, G.variant, cgrid,
 			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, RK_TRACE_NUDGE, st);
 			if (r)
 				return rl_hipfail(r, "extend trace");
@@ -320,7 +329,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 				return rl_hipfail(r, "shade");
 			slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->srays, NULL, t->occ,
-			  n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 0, 0, G.rcp_ieee, G.variant, agrid, G.refill,
+			  n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 0, 0, PT_RCP_IEEE, G.variant, agrid, G.refill,
 			  G.max_iter, &t->counts[PT_CNT_SHADOW], G.diag, RK_TRACE_NUDGE, st);
 			if (r)
 				return rl_hipfail(r, "shadow trace");
@@ -364,7 +373,7 @@ static unsigned long long pt_key(const struct rl_scene *s, const struct PtArgs *
 	  t->spath, t->occ, t->scontrib, t->counts, t->lights, t->lcdf, G.ctl, G.diag,
 	  G.spill};
 	MIX(ptrs, sizeof(ptrs));
-	unsigned long long misc[] = {s->tlas, t->batch, G.refill, (unsigned long long)G.rcp_ieee,
+	unsigned long long misc[] = {s->tlas, t->batch, G.refill, (unsigned long long)PT_RCP_IEEE,
 	  P.shade_mult, (unsigned long long)G.variant, G.grid_mult, s->gen, t->lights_gen,
 	  G.spill_bytes, t->cap};
 	MIX(misc, sizeof(misc));
@@ -429,7 +438,7 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 		a.guide_share = views[0].spp;
 	/* allocations (spill area) happen before the key and any capture */
 	unsigned g0;
-	if ((r = rl_trace_grid(1, 0, &g0)) || (r = rl_trace_grid(0, 0, &g0)))
+	if ((r = rl_trace_grid(1, 0, PT_RCP_IEEE, &g0)) || (r = rl_trace_grid(0, 0, PT_RCP_IEEE, &g0)))
 		return r;
 	unsigned long long key = pt_key(s, &a, views, nv, acc, st);
 	/* Guided frames change the network between calls and run host-scheduled

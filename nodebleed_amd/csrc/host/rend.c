@@ -158,9 +158,9 @@ int rend_hip_get_tuning(unsigned int out[3])
 }
 
 /* Persistent grid for a trace variant and the matching spill area. */
-int rl_trace_grid(int closest, int count, unsigned *grid)
+int rl_trace_grid(int closest, int count, int rcp_ieee, unsigned *grid)
 {
-	int occ = rk_trace_occupancy(closest, count, G.rcp_ieee, G.variant);
+	int occ = rk_trace_occupancy(closest, count, rcp_ieee, G.variant);
 	if (occ < 1)
 		return rl_fail(REND_ENODEV, "trace kernel occupancy query failed");
 	unsigned mult = G.grid_mult ? G.grid_mult : (unsigned)occ;
@@ -843,7 +843,7 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
                    hipStream_t st)
 {
 	unsigned grid;
-	int r = rl_trace_grid(closest, count, &grid);
+	int r = rl_trace_grid(closest, count, G.rcp_ieee, &grid);
 	if (r)
 		return r;
 	if ((r = rl_stream_order(st)))
