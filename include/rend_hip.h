@@ -75,6 +75,19 @@ int          rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
                             struct rhit *d_hits, unsigned int n,
                             void *stream, struct rtrace_stats *stats);
 
+/* rend_hip_trace without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc); it has no counterpart in the reference.
+ * Same kernels and results as rend_hip_trace.  The scene must already be on
+ * the device (rend_hip_sync_scene after rend_prepstatic/rend_prepdynamic,
+ * else REND_ESTATE), n <= 2^31 - 1 rays (else REND_EINVAL), no visit
+ * counting.  The traversal stack spills into a static 48 MiB array per
+ * device that the library holds until the process ends; every call on a
+ * device shares it, so calls that overlap on one device race (calls on
+ * different streams are ordered by the library outside graph capture). */
+int          rend_hip_trace_noalloc(struct rdata *rd, const struct rray *d_rays,
+                                    struct rhit *d_hits, unsigned int n, void *stream);
+
 /* Any hit within (0, tmax): d_occ[i] = 1 when occluded, else 0.  Reference
  * semantics: intersect_any_tlas (rend.c:2883-2998). */
 int          rend_hip_occluded(struct rdata *rd, const struct rray *d_rays,

@@ -88,6 +88,8 @@ _SIGS = {
     "rend_hip_arena_bytes": (C.c_longlong, [C.c_void_p]),
     "rend_hip_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
                                  C.c_void_p, C.POINTER(TraceStats)]),
+    "rend_hip_trace_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
+                                         C.c_void_p]),
     "rend_hip_occluded": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,
                                     C.c_void_p]),
     "rend_hip_trace_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint,

@@ -80,6 +80,14 @@ struct CamArgs {
 #define RK_CTL_ERR     (RK_NSEG * RK_SEG_STRIDE)
 #define RK_CTL_WORDS   (RK_CTL_ERR + RK_SEG_STRIDE)
 
+/* Allocation-free closest-hit launches (rend_hip_trace_noalloc) spill into
+ * a static device array sized for at most RK_NOALLOC_BLOCKS persistent
+ * blocks of the 16-entry LDS stack variant: 512 x 256 threads x 48 entries
+ * x 8 bytes = 48 MiB per device. */
+#define RK_NOALLOC_BLOCKS    512u
+#define RK_NOALLOC_DEPTH     16
+#define RK_NOALLOC_MAX_RAYS  0x7fffffffu
+
 /* Error bits written by kernels into the device status word. */
 /* Diagnostic block shared by every traversal launch (after the control
  * ring): [0] sticky error bits, [1] claim flag, [4..11] the first failing

@@ -37,6 +37,7 @@ struct rl_global {
 	hipEvent_t          order_ev;
 	hipStream_t         order_stream;
 	int                 order_pending;
+	int                 scratch_err;  /* result of the static spill area lookup */
 };
 
 struct rl_frame;  /* frame.c */
@@ -139,6 +140,13 @@ void             rl_camargs(const struct rcam *c, struct CamArgs *a);
 int rk_trace_variants(void);
 int rk_trace_lds_depth(int variant);
 int rk_trace_occupancy(int closest, int count, int rcp_ieee, int variant);
+int rk_scratch_init(void);
+unsigned long long rk_scratch_bytes(void);
+int rk_trace_launch_noalloc(const void *arena, unsigned long long tlas,
+                            const void *insts, const void *rays, void *hits,
+                            unsigned int n, unsigned int *ctl, int rcp_ieee,
+                            unsigned int cus, unsigned int refill,
+                            unsigned int max_iter, unsigned int *diag, void *stream);
 int rk_trace_launch(const void *arena, unsigned long long tlas,
                     const void *insts, const void *rays, void *hits,
                     void *occ, unsigned int n, unsigned int *ctl,

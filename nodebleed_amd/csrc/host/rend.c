@@ -99,6 +99,9 @@ int rl_device(void)
 		pthread_mutex_unlock(&g_lock);
 		return rl_hipfail(e, "launch order event");
 	}
+	/* device address of the allocation-free entry point's static spill
+	 * area (device 0 is current); a failure only disables that entry */
+	G.scratch_err = rk_scratch_init();
 	G.ready = 1;
 	pthread_mutex_unlock(&g_lock);
 	return 0;
@@ -967,6 +970,42 @@ int rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
 	stats->insts += c[2];
 	stats->rays += c[3];
 	return rl_check_errors(st);
+}
+
+/* Closest hit without any allocation: the scene must be on the device
+ * already (rend_hip_sync_scene), the control ring and error block are the
+ * ones rl_device allocated at set-up, and the traversal stack spills into
+ * the static array of kern/trace.hip.  Asynchronous; errors are reported by
+ * the next synchronous call, as for rend_hip_trace without stats. */
+int rend_hip_trace_noalloc(struct rdata *rd, const struct rray *d_rays,
+                           struct rhit *d_hits, unsigned int n, void *stream)
+{
+	if (!G.ready)
+		return rl_fail(REND_ESTATE, "trace_noalloc: device not set up (upload the "
+		  "scene with rend_hip_sync_scene first)");
+	if (G.scratch_err)
+		return rl_hipfail(G.scratch_err, "trace_noalloc: static spill area");
+	struct rl_scene *s = rl_scene(rd);
+	if (!s || !s->tlas_built || !s->dynamic_ok || s->static_dirty || s->dynamic_dirty)
+		return rl_fail(REND_ESTATE, "trace_noalloc: scene not on the device (call "
+		  "rend_hip_sync_scene after rend_prepstatic/rend_prepdynamic)");
+	if (n > RK_NOALLOC_MAX_RAYS)
+		return rl_fail(REND_EINVAL, "trace_noalloc: %u rays, at most %u per call", n,
+		  RK_NOALLOC_MAX_RAYS);
+	if (n && (!d_rays || !d_hits))
+		return rl_fail(REND_EINVAL, "trace_noalloc: no ray or hit buffer");
+	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
+	int r = rl_stream_order(st);
+	if (r)
+		return r;
+	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
+	int e = rk_trace_launch_noalloc(s->arena, s->tlas, s->insts, d_rays, d_hits, n,
+	  G.ctl + RK_CTL_WORDS * slot, G.rcp_ieee, (unsigned)G.cus, G.refill, G.max_iter,
+	  G.diag, st);
+	if (e)
+		return rl_hipfail(e, "trace_noalloc launch");
+	G.last_slot = slot;
+	return rl_stream_mark(st);
 }
 
 int rend_hip_occluded(struct rdata *rd, const struct rray *d_rays,

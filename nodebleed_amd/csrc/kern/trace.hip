@@ -607,3 +607,76 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
   }
   return (int)e;
 }
+
+// ---- allocation-free launches ---------------------------------------------
+//
+// The spill area of rend_hip_trace_noalloc is this file-scope array, not a
+// library allocation.  Host code never passes its name to a kernel: the
+// device address comes from hipGetSymbolAddress, taken once per device at
+// library set-up (rk_scratch_init) and kept per device.  Every call on a
+// device shares its array; calls that overlap on one device race on it.
+
+constexpr size_t kScratchEntries =
+    (size_t)RK_NOALLOC_BLOCKS * REND_TRACE_BLOCK * (REND_STACK_MAX - RK_NOALLOC_DEPTH);
+static __device__ __align__(256) uint2 scratch[kScratchEntries];
+
+constexpr int kMaxDevices = 64;
+static void *g_scratch[kMaxDevices];
+
+// Called at library set-up with the device current.  Returns a hipError_t.
+extern "C" int rk_scratch_init(void) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  if (g_scratch[dev]) return 0;
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return (int)e;
+  g_scratch[dev] = p;
+  return 0;
+}
+
+extern "C" unsigned long long rk_scratch_bytes(void) { return sizeof(scratch); }
+
+// Closest-hit launch of the 16-entry LDS stack variant with the static spill
+// area; grid = min(occupancy x CUs, RK_NOALLOC_BLOCKS).  Allocates nothing.
+extern "C" int rk_trace_launch_noalloc(const void *arena, unsigned long long tlas,
+                                       const void *insts, const void *rays, void *hits,
+                                       unsigned int n, unsigned int *ctl, int rcp_ieee,
+                                       unsigned int cus, unsigned int refill,
+                                       unsigned int max_iter, unsigned int *diag,
+                                       void *stream) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev]) return (int)hipErrorNotInitialized;
+  const int occ = rcp_ieee ? occ_one<true, false, true, RK_NOALLOC_DEPTH, 4>()
+                           : occ_one<true, false, false, RK_NOALLOC_DEPTH, 4>();
+  if (occ < 1) return (int)hipErrorInvalidConfiguration;
+  uint32_t grid = (uint32_t)occ * cus;
+  if (grid > RK_NOALLOC_BLOCKS) grid = RK_NOALLOC_BLOCKS;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_u32, dim3(4), dim3(64), 0, s, ctl, (uint32_t)RK_CTL_ERR);
+  e = hipGetLastError();
+  if (e != hipSuccess || n == 0) return (int)e;
+  LaunchArgs a;
+  a.arena = (const uint8_t *)arena;
+  a.tlas = tlas;
+  a.insts = (const dinst *)insts;
+  a.rays = (const rray *)rays;
+  a.hits = (rhit *)hits;
+  a.occ = nullptr;
+  a.n = n;
+  a.ctl = ctl;
+  a.spill = (uint2 *)g_scratch[dev];
+  a.stats = nullptr;
+  a.refill = refill;
+  a.max_iter = max_iter;
+  a.n_dev = nullptr;
+  a.diag = diag;
+  a.flags = 0;
+  e = rcp_ieee ? launch_one<true, false, true, RK_NOALLOC_DEPTH, 4>(grid, s, a)
+               : launch_one<true, false, false, RK_NOALLOC_DEPTH, 4>(grid, s, a);
+  return (int)e;
+}
