@@ -105,6 +105,16 @@ int          rend_hip_occluded_host(struct rdata *rd, const struct rray *rays,
  * the default) and the number of idle lanes that triggers a wave refill
  * (1..64, default 48). */
 int          rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill);
+/* Lanes of a wave that wait at leaves (or at instance entries and ray
+ * set-ups) before the wave runs that step while other lanes still have
+ * interior nodes to visit (1..64; 0 = the default, 8).  Results never
+ * depend on it: each ray's visits keep their order. */
+int          rend_hip_set_step_batch(unsigned int lanes);
+/* Wave-level counters of the last counting trace (rend_hip_trace with
+ * stats): wave loop iterations, lanes holding a ray summed over those
+ * iterations (lane utilisation = out[1] / (64 out[0])), leaf steps and
+ * set-up steps run. */
+int          rend_hip_trace_lanes(unsigned long long out[4]);
 
 /* Traversal kernel variant (0..3): LDS stack depth / waves per SIMD of
  * 16/4, 12/5, 10/6 and 8/8 (deeper entries spill to HBM).  Results are

@@ -79,6 +79,8 @@ _SIGS = {
                                          C.POINTER(C.c_uint)]),
     "rend_hip_set_tuning": (C.c_int, [C.c_uint, C.c_uint]),
     "rend_hip_set_trace_variant": (C.c_int, [C.c_int]),
+    "rend_hip_set_step_batch": (C.c_int, [C.c_uint]),
+    "rend_hip_trace_lanes": (C.c_int, [C.c_void_p]),
     "rend_hip_set_iter_limit": (C.c_int, [C.c_uint]),
     "rend_hip_get_tuning": (C.c_int, [C.c_void_p]),
     "rend_hip_set_mode": (C.c_int, [C.c_int]),

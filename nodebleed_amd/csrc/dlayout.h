@@ -67,6 +67,9 @@ struct CamArgs {
 /* rk_trace_launch flags: replace exactly-zero instance-space direction
    components by a signed 1e-20 (path-tracing callers only) */
 #define RK_TRACE_NUDGE      1u
+/* bits 8..15 of the flags: lanes a leaf or setup step waits for (0: the
+   default RK_STEP_BATCH) */
+#define RK_STEP_BATCH       8u
 #define REND_TRACE_BLOCK 256
 
 /* Per-launch control block of the traversal kernel: RK_NSEG ray segments,

@@ -22,12 +22,14 @@ struct rl_global {
 	unsigned            max_iter;    /* traversal step limit per ray */
 	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
 	int                 variant;     /* traversal occupancy variant */
+	unsigned            step_batch;  /* lanes a leaf/setup step waits for, 0 = default */
 	int                 tlas_gpu;    /* rend_prepdynamic builds the TLAS on the device */
 	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS, then RK_DIAG_WORDS */
 	unsigned           *diag;        /* error block shared by all traversal launches */
 	unsigned            ctl_next;
 	unsigned            last_slot;
-	unsigned long long *stats;       /* 4 counters of the counting variant */
+	unsigned long long *stats;       /* 8 counters of the counting variant */
+	unsigned long long  lanes[4];    /* wave-level counters of the last counting launch */
 	void               *spill;       /* traversal stack spill area */
 	size_t              spill_bytes;
 	/* Every traversal launch and path-tracing frame shares the control
@@ -114,6 +116,7 @@ int              rl_trace_async(struct rl_scene *s, const void *d_rays,
                                 int closest, int count, hipStream_t st);
 int              rl_check_errors(hipStream_t st);
 int              rl_stream_order(hipStream_t st);
+unsigned         rl_trace_flags(unsigned extra);
 int              rl_stream_mark(hipStream_t st);
 unsigned         rl_blkszx(void);
 unsigned         rl_blkszy(void);

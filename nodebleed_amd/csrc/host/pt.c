@@ -316,7 +316,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 			unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->rays[cur], t->hits,
 			  NULL, n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 1, 0, PT_RCP_IEEE, G.variant, cgrid,
-			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, RK_TRACE_NUDGE, st);
+			  G.refill, G.max_iter, &t->counts[PT_CNT_CUR], G.diag, rl_trace_flags(RK_TRACE_NUDGE), st);
 			if (r)
 				return rl_hipfail(r, "extend trace");
 			if (mix && d < a->guide_depth && (r = rl_guide_eval(s, a, t->rays[cur], t->hits,
@@ -330,7 +330,7 @@ static int pt_launches(struct rl_scene *s, const struct PtArgs *a,
 			slot = G.ctl_next++ % RL_CTL_SLOTS;
 			r = rk_trace_launch(s->arena, s->tlas, s->insts, t->srays, NULL, t->occ,
 			  n, G.ctl + RK_CTL_WORDS * slot, G.spill, NULL, 0, 0, PT_RCP_IEEE, G.variant, agrid, G.refill,
-			  G.max_iter, &t->counts[PT_CNT_SHADOW], G.diag, RK_TRACE_NUDGE, st);
+			  G.max_iter, &t->counts[PT_CNT_SHADOW], G.diag, rl_trace_flags(RK_TRACE_NUDGE), st);
 			if (r)
 				return rl_hipfail(r, "shadow trace");
 			G.last_slot = slot;
@@ -375,7 +375,7 @@ static unsigned long long pt_key(const struct rl_scene *s, const struct PtArgs *
 	MIX(ptrs, sizeof(ptrs));
 	unsigned long long misc[] = {s->tlas, t->batch, G.refill, (unsigned long long)PT_RCP_IEEE,
 	  P.shade_mult, (unsigned long long)G.variant, G.grid_mult, s->gen, t->lights_gen,
-	  G.spill_bytes, t->cap};
+	  G.spill_bytes, t->cap, G.step_batch};
 	MIX(misc, sizeof(misc));
 #undef MIX
 	return h;

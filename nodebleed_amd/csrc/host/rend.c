@@ -89,7 +89,7 @@ int rl_device(void)
 	const size_t ctl_words = RL_CTL_SLOTS * RK_CTL_WORDS + RK_DIAG_WORDS;
 	if ((e = hipMalloc((void **)&G.ctl, ctl_words * sizeof(unsigned))) != hipSuccess ||
 	    (e = hipMemset(G.ctl, 0, ctl_words * sizeof(unsigned))) != hipSuccess ||
-	    (e = hipMalloc((void **)&G.stats, 4 * sizeof(unsigned long long)))
+	    (e = hipMalloc((void **)&G.stats, 8 * sizeof(unsigned long long)))
 	      != hipSuccess) {
 		pthread_mutex_unlock(&g_lock);
 		return rl_hipfail(e, "control words");
@@ -150,6 +150,26 @@ int rend_hip_set_trace_variant(int variant)
 		return rl_fail(REND_EINVAL, "trace variant %d", variant);
 	G.variant = variant;
 	return 0;
+}
+
+int rend_hip_trace_lanes(unsigned long long out[4])
+{
+	memcpy(out, G.lanes, sizeof(G.lanes));
+	return 0;
+}
+
+int rend_hip_set_step_batch(unsigned int lanes)
+{
+	if (lanes > 64)
+		return rl_fail(REND_EINVAL, "step batch %u (0..64)", lanes);
+	G.step_batch = lanes;
+	return 0;
+}
+
+/* Launch flags of every traversal launch: bits 8..15 carry the step batch. */
+unsigned rl_trace_flags(unsigned extra)
+{
+	return extra | (G.step_batch << 8);
 }
 
 int rend_hip_get_tuning(unsigned int out[3])
@@ -854,7 +874,8 @@ int rl_trace_async(struct rl_scene *s, const void *d_rays, void *d_hits,
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch(s->arena, s->tlas, s->insts, d_rays, d_hits,
 	  d_occ, n, G.ctl + RK_CTL_WORDS * slot, G.spill, count ? G.stats : NULL,
-	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, G.diag, 0, st);
+	  closest, count, G.rcp_ieee, G.variant, grid, G.refill, G.max_iter, NULL, G.diag,
+	  rl_trace_flags(0), st);
 	if (e)
 		return rl_hipfail(e, "trace launch");
 	G.last_slot = slot;
@@ -951,14 +972,14 @@ int rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
 		if ((r = rl_stream_order(st)))
 			return r;
 		hipError_t e = hipMemsetAsync(G.stats, 0,
-		  4 * sizeof(unsigned long long), st);
+		  8 * sizeof(unsigned long long), st);
 		if (e != hipSuccess)
 			return rl_hipfail(e, "stats clear");
 	}
 	r = rl_trace_async(s, d_rays, d_hits, NULL, n, 1, stats != NULL, st);
 	if (r || !stats)
 		return r;
-	unsigned long long c[4];
+	unsigned long long c[8];
 	hipError_t e = hipMemcpyAsync(c, G.stats, sizeof(c), hipMemcpyDeviceToHost,
 	  st);
 	if (e == hipSuccess)
@@ -969,6 +990,7 @@ int rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
 	stats->leaves += c[1];
 	stats->insts += c[2];
 	stats->rays += c[3];
+	memcpy(G.lanes, &c[4], sizeof(G.lanes));
 	return rl_check_errors(st);
 }
 

@@ -196,7 +196,20 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     if (i >= (uint32_t)DEPTH) myspill[(i - DEPTH) * sstride] = e;
   };
 
-  bool active = false;
+  // Lane states.  A lane waits in its state until the wave runs that
+  // state's step; every ray still makes exactly the reference's sequence of
+  // visits, only the interleaving across lanes changes:
+  //   NODE        ofs is an interior node of the current tree
+  //   LEAF        ofs is a BLAS leaf
+  //   SETUP_INST  ofs is a TLAS leaf: transform the world ray, enter the BLAS
+  //   SETUP_WORLD the world ray's slab constants are needed (a new ray at the
+  //               TLAS root, or a TLAS node popped after a BLAS)
+  enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_SETUP_INST = 3,
+                    ST_SETUP_WORLD = 4 };
+  // leaf and setup steps run for at least this many waiting lanes, or once
+  // fewer lanes than this are left at interior nodes
+  const uint32_t kStepBatch = (flags >> 8) & 0xffu ? (flags >> 8) & 0xffu : RK_STEP_BATCH;
+  uint32_t state = ST_IDLE;
   bool exhausted = false;  // wave-uniform
   // wave-uniform claim state: rays [lnext, lend) are claimed but unassigned
   uint32_t lnext = 0, lend = 0, tried = 0;
@@ -211,10 +224,255 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   bool in_blas = false;
   bool occluded = false;
   uint64_t c_nodes = 0, c_leaves = 0, c_insts = 0, c_rays = 0;
+  uint64_t c_witer = 0, c_lanes = 0, c_leafx = 0, c_setupx = 0;
+
+  // next state for a lane at entry ofs of the current tree
+  auto classify = [&](uint32_t o) -> uint32_t {
+    return !(o & DNODE_LEAF) ? ST_NODE : (in_blas ? ST_LEAF : ST_SETUP_INST);
+  };
 
   for (;;) {
+    const uint64_t m_live = __ballot(state != ST_IDLE);
+    if (COUNT && m_live) {
+      c_witer++;
+      c_lanes += (uint64_t)__popcll(m_live);
+    }
+
+    bool need_pop = false;
+    bool done = false;
+
+    // ---- node step -------------------------------------------------------
+    const uint64_t m_node = __ballot(state == ST_NODE);
+    if (m_node && state == ST_NODE) {
+      if (++iter > max_iter) {
+        report_error(diag, RK_ERR_ITER, rays, rid, iter);
+        done = true;
+      } else {
+        const dnode *nd = (const dnode *)(arena + base + ofs);
+        float tmn[8];
+        uint32_t ch[8], invw;
+        // the path is chosen before the node is read, so each is one block
+        const uint32_t hmr = __ballot(!(cr.fin && t == t) && state == ST_NODE) == 0
+                                 ? slab8<true>(nd, cr, t, tmn, ch, invw)
+                                 : slab8<false>(nd, cr, t, tmn, ch, invw);
+        if (COUNT) c_nodes++;
+
+        const uint32_t hitcnt = __builtin_popcount(hmr);
+        if (hitcnt == 0) {
+          need_pop = true;
+        } else if (sp + hitcnt > REND_STACK_MAX) {
+          // The reference asserts here (rend.c:834); report instead of
+          // writing past the stack.
+          report_error(diag, RK_ERR_STACK, rays, rid, iter);
+          done = true;
+        } else {
+          // Highest rank among hits (nearest) is the next node, the others
+          // are pushed farthest first.  The any-hit pass uses the same order:
+          // its t_far never shrinks, so the set of leaves it can reach, and
+          // with it the occlusion result, does not depend on the visiting
+          // order (the reference pushes unordered, rend.c:1884-1886).
+          uint32_t next = 0;
+          const uint32_t keym = hmr;
+          const uint32_t top = 31u - __builtin_clz(keym);
+          if (__ballot(sp + 7u > (uint32_t)DEPTH && state == ST_NODE) == 0) {
+            // no lane can reach the spill area: branch-free pushes,
+            // children that are not kept go to the junk row
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+              const uint32_t key = (invw >> (3 * c)) & 7u;
+              const bool hit = (keym >> key) & 1u;
+              const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
+              const uint32_t slot = (hit && key != top) ? sp + pos : (uint32_t)DEPTH;
+              s_stack[slot][tid] =
+                  make_uint2(ch[c], CLOSEST ? __float_as_uint(tmn[c]) : 0u);
+              next = key == top ? ch[c] : next;
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+              const uint32_t key = (invw >> (3 * c)) & 7u;
+              if ((keym >> key) & 1u) {
+                if (key == top) {
+                  next = ch[c];
+                } else {
+                  const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
+                  put(sp + pos, ch[c], CLOSEST ? tmn[c] : 0.0f);
+                }
+              }
+            }
+          }
+          sp += hitcnt - 1;
+          ofs = next;
+          state = classify(next);
+        }
+      }
+    }
+
+    // ---- leaf step: run when enough lanes wait at leaves, or when few
+    // lanes are left at interior nodes --------------------------------------
+    const uint64_t m_leaf = __ballot(state == ST_LEAF && !need_pop && !done);
+    // lanes that stay in the node phase: at an interior node, or about to
+    // pop (most pops lead to another interior node)
+    const uint64_t m_node2 = __ballot((state == ST_NODE || need_pop) && !done);
+    const uint32_t n_node = (uint32_t)__popcll(m_node2);
+    if (m_leaf && ((uint32_t)__popcll(m_leaf) >= kStepBatch || n_node < kStepBatch)) {
+      if (COUNT) c_leafx++;
+      if (state == ST_LEAF && !need_pop && !done) {
+        if (++iter > max_iter) {
+          report_error(diag, RK_ERR_ITER, rays, rid, iter);
+          done = true;
+        } else {
+          const dleaf *lf = (const dleaf *)(arena + base + (ofs & ~DNODE_LEAF));
+          float4 v0x = *(const float4 *)lf->v0x, v0y = *(const float4 *)lf->v0y,
+                 v0z = *(const float4 *)lf->v0z;
+          float4 e0x = *(const float4 *)lf->e0x, e0y = *(const float4 *)lf->e0y,
+                 e0z = *(const float4 *)lf->e0z;
+          float4 e1x = *(const float4 *)lf->e1x, e1y = *(const float4 *)lf->e1y,
+                 e1z = *(const float4 *)lf->e1z;
+          const float *V0x = (const float *)&v0x, *V0y = (const float *)&v0y,
+                      *V0z = (const float *)&v0z;
+          const float *E0x = (const float *)&e0x, *E0y = (const float *)&e0y,
+                      *E0z = (const float *)&e0z;
+          const float *E1x = (const float *)&e1x, *E1y = (const float *)&e1y,
+                      *E1z = (const float *)&e1z;
+          if (COUNT) c_leaves++;
+
+          float tt[4], uu[4], vv[4];
+          uint32_t mask = 0;
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const float pvx = __builtin_fmaf(cr.dy, E1z[k], -(cr.dz * E1y[k]));
+            const float pvy = __builtin_fmaf(cr.dz, E1x[k], -(cr.dx * E1z[k]));
+            const float pvz = __builtin_fmaf(cr.dx, E1y[k], -(cr.dy * E1x[k]));
+            const float tvx = cr.ox - V0x[k];
+            const float tvy = cr.oy - V0y[k];
+            const float tvz = cr.oz - V0z[k];
+            const float det = __builtin_fmaf(E0x[k], pvx,
+                              __builtin_fmaf(E0y[k], pvy, E0z[k] * pvz));
+            const float qvx = __builtin_fmaf(tvy, E0z[k], -(tvz * E0y[k]));
+            const float qvy = __builtin_fmaf(tvz, E0x[k], -(tvx * E0z[k]));
+            const float qvz = __builtin_fmaf(tvx, E0y[k], -(tvy * E0x[k]));
+            const float idet = RCP_IEEE ? 1.0f / det : rcp_table(det, s_tab);
+            const float u = idet * __builtin_fmaf(tvx, pvx, __builtin_fmaf(tvy, pvy, tvz * pvz));
+            const float v = idet * __builtin_fmaf(cr.dx, qvx, __builtin_fmaf(cr.dy, qvy, cr.dz * qvz));
+            const float tk = idet * __builtin_fmaf(E1x[k], qvx, __builtin_fmaf(E1y[k], qvy, E1z[k] * qvz));
+            const bool ok = (u >= 0.0f) & (v >= 0.0f) & ((u + v) <= 1.0f) &
+                            (tk > 0.0f) & (tk < t);
+            tt[k] = tk; uu[k] = u; vv[k] = v;
+            mask |= (ok ? 1u : 0u) << k;
+          }
+          if (!CLOSEST) {
+            if (mask) { done = true; occluded = true; }
+            else need_pop = true;
+          } else {
+            if (mask) {
+              float tb[4];
+#pragma unroll
+              for (int k = 0; k < 4; k++) tb[k] = ((mask >> k) & 1u) ? tt[k] : FLT_MAXF;
+              const float m = xmin(xmin(tb[0], tb[1]), xmin(tb[2], tb[3]));
+              int sel = 0;
+#pragma unroll
+              for (int k = 0; k < 4; k++) if (tb[k] == m) sel = k;
+              float su = uu[0], sv = vv[0], stt = tb[0];
+#pragma unroll
+              for (int k = 1; k < 4; k++)
+                if (sel == k) { su = uu[k]; sv = vv[k]; stt = tb[k]; }
+              t = stt;
+              tu = su;
+              tv = sv;
+              hid = (lf->id[sel] << 12) | instid;
+            }
+            need_pop = true;
+          }
+        }
+      }
+    }
+
+    // ---- setup step: instance entry (transform, then the BLAS ray's slab
+    // constants) and world-ray constants (new rays, TLAS re-entry) ----------
+    const uint64_t m_setup =
+        __ballot((state == ST_SETUP_INST || state == ST_SETUP_WORLD) && !need_pop && !done);
+    if (m_setup && ((uint32_t)__popcll(m_setup) >= kStepBatch || n_node < kStepBatch)) {
+      if (COUNT) c_setupx++;
+      if ((state == ST_SETUP_INST || state == ST_SETUP_WORLD) && !need_pop && !done) {
+        if (state == ST_SETUP_WORLD) {
+          ray_setup(cr, wo[0], wo[1], wo[2], wd[0], wd[1], wd[2]);
+          state = ST_NODE;
+        } else if (++iter > max_iter) {
+          report_error(diag, RK_ERR_ITER, rays, rid, iter);
+          done = true;
+        } else {
+          // TLAS leaf: enter the instance's BLAS with the transformed ray.
+          instid = ofs & ~DNODE_LEAF;
+          const dinst *in = insts + instid;
+          const float4 m0 = *(const float4 *)&in->inv[0];
+          const float4 m1 = *(const float4 *)&in->inv[4];
+          const float4 m2 = *(const float4 *)&in->inv[8];
+          const uint64_t blas = in->blas;
+          if (COUNT) c_insts++;
+          const float ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
+          const float oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
+          const float oz = m2.x * wo[0] + m2.y * wo[1] + m2.z * wo[2] + m2.w;
+          const float dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
+          const float dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
+          const float dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
+          if (flags & RK_TRACE_NUDGE)
+            ray_setup(cr, ox, oy, oz, nudge0(dx), nudge0(dy), nudge0(dz));
+          else
+            ray_setup(cr, ox, oy, oz, dx, dy, dz);
+          in_blas = true;
+          blas_sp = sp;
+          base = blas;
+          ofs = 0;
+          state = ST_NODE;
+        }
+      }
+    }
+
+    // ---- pops: after a node without hits or a leaf -------------------------
+    if (need_pop && !done) {
+      bool world = false;
+      for (;;) {
+        if (in_blas && sp == blas_sp) {
+          // back in the TLAS: the world ray's constants are needed again
+          // before the next TLAS node (not for an instance entry or the end)
+          in_blas = false;
+          base = tlas;
+          world = true;
+        }
+        if (sp == 0) { done = true; break; }
+        --sp;
+        uint2 e = s_stack[sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH][tid];
+        if (sp >= (uint32_t)DEPTH) {
+          // volatile keeps the compiler from merging this load with the LDS
+          // one into a single generic (flat) load
+          const volatile uint32_t *q = (const volatile uint32_t *)&myspill[(sp - DEPTH) * sstride];
+          e.x = q[0];
+          e.y = q[1];
+        }
+        if (!CLOSEST || __uint_as_float(e.y) <= t) { ofs = e.x; break; }
+      }
+      if (!done) {
+        const uint32_t nst = classify(ofs);
+        state = (world && nst == ST_NODE) ? ST_SETUP_WORLD : nst;
+      }
+    }
+
+    if (done) {
+      if (CLOSEST) {
+        rhit h; h.t = t; h.u = tu; h.v = tv; h.id = hid;
+        *(float4 *)&hits[rid] = *(const float4 *)&h;
+      } else {
+        occ[rid] = occluded ? 1 : 0;
+      }
+      state = ST_IDLE;
+    }
+
     // ---- refill idle lanes from the wave's claimed chunk ------------------
-    const uint64_t idle = __ballot(!active);
+    // At the end of the loop body: a new ray's record is loaded here and its
+    // slab constants computed by the next iteration's SETUP_WORLD step, after
+    // the node step, so the load's latency overlaps the other lanes' work.
+    const uint64_t idle = __ballot(state == ST_IDLE);
     const uint32_t nidle = (uint32_t)__popcll(idle);
     if (!exhausted && nidle >= refill) {
       // claim a new chunk when the current one is used up (one atomic per
@@ -234,7 +492,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         }
       }
       const uint32_t take = min(nidle, lend - lnext);
-      if (!active) {
+      if (state == ST_IDLE) {
         const uint32_t below = __builtin_amdgcn_mbcnt_hi(
             (uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
         const uint32_t r = lnext + below;
@@ -244,7 +502,6 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           const float4 b = *(const float4 *)&rays[r].dx;
           wo[0] = a.x; wo[1] = a.y; wo[2] = a.z;
           wd[0] = b.x; wd[1] = b.y; wd[2] = b.z;
-          ray_setup(cr, a.x, a.y, a.z, b.x, b.y, b.z);
           t = a.w;
           tu = tv = 0.0f;
           hid = 0xffffffffu;
@@ -252,199 +509,13 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           base = tlas;
           in_blas = false;
           occluded = false;
-          active = true;
+          state = ST_SETUP_WORLD;
           if (COUNT) c_rays++;
         }
       }
       lnext += take;
     }
-    if (__ballot(active) == 0) {
-      if (exhausted) break;
-      continue;
-    }
-    if (!active) continue;
-
-    // ---- one traversal step -------------------------------------------------
-    bool need_pop = false;
-    bool done = false;
-    if (++iter > max_iter) {
-      report_error(diag, RK_ERR_ITER, rays, rid, iter);
-      done = true;
-    } else if (!(ofs & DNODE_LEAF)) {
-      const dnode *nd = (const dnode *)(arena + base + ofs);
-      float tmn[8];
-      uint32_t ch[8], invw;
-      // the path is chosen before the node is read, so each is one block
-      const uint32_t hmr = __ballot(!(cr.fin && t == t)) == 0
-                               ? slab8<true>(nd, cr, t, tmn, ch, invw)
-                               : slab8<false>(nd, cr, t, tmn, ch, invw);
-      if (COUNT) c_nodes++;
-
-      const uint32_t hitcnt = __builtin_popcount(hmr);
-      if (hitcnt == 0) {
-        need_pop = true;
-      } else if (sp + hitcnt > REND_STACK_MAX) {
-        // The reference asserts here (rend.c:834); report instead of writing
-        // past the stack.
-        report_error(diag, RK_ERR_STACK, rays, rid, iter);
-        done = true;
-      } else {
-        // Highest rank among hits (nearest) is the next node, the others are
-        // pushed farthest first.  The any-hit pass uses the same order: its
-        // t_far never shrinks, so the set of leaves it can reach, and with it
-        // the occlusion result, does not depend on the visiting order (the
-        // reference pushes unordered, rend.c:1884-1886).
-        uint32_t next = 0;
-        const uint32_t keym = hmr;
-        const uint32_t top = 31u - __builtin_clz(keym);
-        if (__ballot(sp + 7u > (uint32_t)DEPTH) == 0) {
-          // no lane can reach the spill area: branch-free pushes, children
-          // that are not kept go to the junk row
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            const uint32_t key = (invw >> (3 * c)) & 7u;
-            const bool hit = (keym >> key) & 1u;
-            const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
-            const uint32_t slot = (hit && key != top) ? sp + pos : (uint32_t)DEPTH;
-            s_stack[slot][tid] =
-                make_uint2(ch[c], CLOSEST ? __float_as_uint(tmn[c]) : 0u);
-            next = key == top ? ch[c] : next;
-          }
-        } else {
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            const uint32_t key = (invw >> (3 * c)) & 7u;
-            if ((keym >> key) & 1u) {
-              if (key == top) {
-                next = ch[c];
-              } else {
-                const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
-                put(sp + pos, ch[c], CLOSEST ? tmn[c] : 0.0f);
-              }
-            }
-          }
-        }
-        sp += hitcnt - 1;
-        ofs = next;
-      }
-    } else if (in_blas) {
-      const dleaf *lf = (const dleaf *)(arena + base + (ofs & ~DNODE_LEAF));
-      float4 v0x = *(const float4 *)lf->v0x, v0y = *(const float4 *)lf->v0y,
-             v0z = *(const float4 *)lf->v0z;
-      float4 e0x = *(const float4 *)lf->e0x, e0y = *(const float4 *)lf->e0y,
-             e0z = *(const float4 *)lf->e0z;
-      float4 e1x = *(const float4 *)lf->e1x, e1y = *(const float4 *)lf->e1y,
-             e1z = *(const float4 *)lf->e1z;
-      const float *V0x = (const float *)&v0x, *V0y = (const float *)&v0y,
-                  *V0z = (const float *)&v0z;
-      const float *E0x = (const float *)&e0x, *E0y = (const float *)&e0y,
-                  *E0z = (const float *)&e0z;
-      const float *E1x = (const float *)&e1x, *E1y = (const float *)&e1y,
-                  *E1z = (const float *)&e1z;
-      if (COUNT) c_leaves++;
-
-      float tt[4], uu[4], vv[4];
-      uint32_t mask = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const float pvx = __builtin_fmaf(cr.dy, E1z[k], -(cr.dz * E1y[k]));
-        const float pvy = __builtin_fmaf(cr.dz, E1x[k], -(cr.dx * E1z[k]));
-        const float pvz = __builtin_fmaf(cr.dx, E1y[k], -(cr.dy * E1x[k]));
-        const float tvx = cr.ox - V0x[k];
-        const float tvy = cr.oy - V0y[k];
-        const float tvz = cr.oz - V0z[k];
-        const float det = __builtin_fmaf(E0x[k], pvx,
-                          __builtin_fmaf(E0y[k], pvy, E0z[k] * pvz));
-        const float qvx = __builtin_fmaf(tvy, E0z[k], -(tvz * E0y[k]));
-        const float qvy = __builtin_fmaf(tvz, E0x[k], -(tvx * E0z[k]));
-        const float qvz = __builtin_fmaf(tvx, E0y[k], -(tvy * E0x[k]));
-        const float idet = RCP_IEEE ? 1.0f / det : rcp_table(det, s_tab);
-        const float u = idet * __builtin_fmaf(tvx, pvx, __builtin_fmaf(tvy, pvy, tvz * pvz));
-        const float v = idet * __builtin_fmaf(cr.dx, qvx, __builtin_fmaf(cr.dy, qvy, cr.dz * qvz));
-        const float tk = idet * __builtin_fmaf(E1x[k], qvx, __builtin_fmaf(E1y[k], qvy, E1z[k] * qvz));
-        const bool ok = (u >= 0.0f) & (v >= 0.0f) & ((u + v) <= 1.0f) &
-                        (tk > 0.0f) & (tk < t);
-        tt[k] = tk; uu[k] = u; vv[k] = v;
-        mask |= (ok ? 1u : 0u) << k;
-      }
-      if (!CLOSEST) {
-        if (mask) { done = true; occluded = true; }
-        else need_pop = true;
-      } else {
-        if (mask) {
-          float tb[4];
-#pragma unroll
-          for (int k = 0; k < 4; k++) tb[k] = ((mask >> k) & 1u) ? tt[k] : FLT_MAXF;
-          const float m = xmin(xmin(tb[0], tb[1]), xmin(tb[2], tb[3]));
-          int sel = 0;
-#pragma unroll
-          for (int k = 0; k < 4; k++) if (tb[k] == m) sel = k;
-          float su = uu[0], sv = vv[0], stt = tb[0];
-#pragma unroll
-          for (int k = 1; k < 4; k++)
-            if (sel == k) { su = uu[k]; sv = vv[k]; stt = tb[k]; }
-          t = stt;
-          tu = su;
-          tv = sv;
-          hid = (lf->id[sel] << 12) | instid;
-        }
-        need_pop = true;
-      }
-    } else {
-      // TLAS leaf: enter the instance's BLAS with the transformed ray.
-      instid = ofs & ~DNODE_LEAF;
-      const dinst *in = insts + instid;
-      const float4 m0 = *(const float4 *)&in->inv[0];
-      const float4 m1 = *(const float4 *)&in->inv[4];
-      const float4 m2 = *(const float4 *)&in->inv[8];
-      const uint64_t blas = in->blas;
-      if (COUNT) c_insts++;
-      const float ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
-      const float oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
-      const float oz = m2.x * wo[0] + m2.y * wo[1] + m2.z * wo[2] + m2.w;
-      const float dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
-      const float dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
-      const float dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
-      if (flags & RK_TRACE_NUDGE)
-        ray_setup(cr, ox, oy, oz, nudge0(dx), nudge0(dy), nudge0(dz));
-      else
-        ray_setup(cr, ox, oy, oz, dx, dy, dz);
-      in_blas = true;
-      blas_sp = sp;
-      base = blas;
-      ofs = 0;
-    }
-
-    if (need_pop && !done) {
-      for (;;) {
-        if (in_blas && sp == blas_sp) {
-          in_blas = false;
-          ray_setup(cr, wo[0], wo[1], wo[2], wd[0], wd[1], wd[2]);
-          base = tlas;
-        }
-        if (sp == 0) { done = true; break; }
-        --sp;
-        uint2 e = s_stack[sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH][tid];
-        if (sp >= (uint32_t)DEPTH) {
-          // volatile keeps the compiler from merging this load with the LDS
-          // one into a single generic (flat) load
-          const volatile uint32_t *q = (const volatile uint32_t *)&myspill[(sp - DEPTH) * sstride];
-          e.x = q[0];
-          e.y = q[1];
-        }
-        if (!CLOSEST || __uint_as_float(e.y) <= t) { ofs = e.x; break; }
-      }
-    }
-
-    if (done) {
-      if (CLOSEST) {
-        rhit h; h.t = t; h.u = tu; h.v = tv; h.id = hid;
-        *(float4 *)&hits[rid] = *(const float4 *)&h;
-      } else {
-        occ[rid] = occluded ? 1 : 0;
-      }
-      active = false;
-    }
+    if (__ballot(state != ST_IDLE) == 0 && exhausted) break;
   }
 
   if (COUNT) {
@@ -459,6 +530,11 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       atomicAdd(&stats[1], (unsigned long long)c_leaves);
       atomicAdd(&stats[2], (unsigned long long)c_insts);
       atomicAdd(&stats[3], (unsigned long long)c_rays);
+      // wave-level figures are the same in every lane: lane 0's alone
+      atomicAdd(&stats[4], (unsigned long long)c_witer);
+      atomicAdd(&stats[5], (unsigned long long)c_lanes);
+      atomicAdd(&stats[6], (unsigned long long)c_leafx);
+      atomicAdd(&stats[7], (unsigned long long)c_setupx);
     }
   }
 }

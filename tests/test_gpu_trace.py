@@ -99,6 +99,32 @@ def test_occupancy_variants_bit_exact(scenes, variant):
         L.rend_hip_set_trace_variant(0)
 
 
+@pytest.mark.parametrize("refill,batch", [(1, 1), (16, 64), (64, 4), (48, 0)])
+def test_wave_scheduling_does_not_change_results(scenes, refill, batch):
+    """Idle lanes refilled at 1..64 idle lanes, leaf and set-up steps run for
+    1..64 waiting lanes: each ray keeps the reference's order of visits, so
+    hits and visit counts are those of the oracle for every setting."""
+    L = lib()
+    s = scenes["instances"]
+    rays = aimed_rays(s, 40_000, seed=31)
+    want, cnt = or_trace(s, rays)
+    check(L.rend_hip_set_tuning(0, refill))
+    check(L.rend_hip_set_step_batch(batch))
+    try:
+        say(f"scheduling refill {refill} step batch {batch}")
+        got, st = s.trace(rays, stats=True)
+        lanes = (C.c_ulonglong * 4)()
+        check(L.rend_hip_trace_lanes(lanes))
+    finally:
+        check(L.rend_hip_set_tuning(0, 48))
+        check(L.rend_hip_set_step_batch(0))
+    assert mismatches(got, want) == {"id": 0, "t": 0, "u": 0, "v": 0}
+    assert (st.nodes, st.leaves, st.insts) == cnt
+    # lane utilisation counters: iterations ran, lanes never above 64 each
+    assert lanes[0] > 0 and 0 < lanes[1] <= 64 * lanes[0]
+    assert lanes[2] > 0 and lanes[3] > 0
+
+
 def test_cube_cracks_match_the_reference_semantics():
     """Camera rays through a two-triangles-per-face cube: with the host RCPSS
     reciprocal about 1 % slip through the crack along a face diagonal and hit
