@@ -6,7 +6,11 @@ instance, 1920x1080 jittered primary rays and one cosine-hemisphere diffuse
 ray per primary hit.  Rays are generated on the GPU, so inputs are resident
 in HBM before timing starts.  A step is one closest-hit launch over the
 2,073,600 incoherent (diffuse) rays; the primary-ray launch is timed beside
-it and reported as an extra field.
+it and reported as an extra field.  The same run then measures the
+path-guiding target on the occluded-lights scene (1920x1080): the wall time
+guided path tracing needs to reach plain path tracing's relMSE after 1,024
+frames (16,384 spp), over plain's time, added to the line as
+"guided_vs_plain" (about four minutes; --no-guide leaves it out).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
 
@@ -341,13 +345,13 @@ def rel_mse(x, r):
     return float(((x - r) ** 2).sum() / (r ** 2).sum())
 
 
-def bench_guide(a):
+def guide_equal_error(a):
     """Equal-error wall time on the occluded-lights scene: plain path tracing
     runs `guide_frames` frames (time T_p, relMSE E_p against a high-spp plain
     GPU reference); guided path tracing (trained online from its own paths,
     starting from an untrained network) runs until its relMSE first reaches
     E_p (time T_g, checked every few frames with the check itself excluded
-    from the clock).  Value: T_g / T_p."""
+    from the clock).  Returns the result record; ratio = T_g / T_p."""
     from nodebleed_amd import REND_MODE_PATHTRACE
     L = lib()
     w, h = (W, H) if not a.guide_small else (480, 270)
@@ -401,6 +405,14 @@ def bench_guide(a):
                 log(f"  {tag} frame {k}/{nframes} {el:.1f} s")
         return el, download(k * spf), curve
 
+    if a.guide_profile_frames:
+        # guided frames only (a profile of their kernels), no error protocol
+        el, _, _ = run(a.guide_profile_frames, True, tag="guided")
+        log(f"guided: {a.guide_profile_frames} frames in {el:.2f} s "
+            f"({el * 1e3 / a.guide_profile_frames:.1f} ms per frame)")
+        L.rend_hip_guide_disable(scene.rd)
+        scene.close()
+        return {"profile_frames": a.guide_profile_frames, "seconds": round(el, 3)}
     t = time.perf_counter()
     _, ref, _ = run(a.guide_ref_frames, False, sample0=1 << 24, tag="reference")
     log(f"reference: {a.guide_ref_frames * spf} spp plain in {time.perf_counter() - t:.1f} s")
@@ -419,22 +431,32 @@ def bench_guide(a):
     log(f"guided: relMSE {err_g:.4f} after {tg:.2f} s ({'reached' if reached else 'not reached'}"
         f" plain's {err_p:.4f}); ratio {ratio:.3f}")
     L.rend_hip_guide_disable(scene.rd)
-    emit("guided/plain wall time to equal relMSE", ratio, "ratio", a.guide_frames, 0,
-         tp * 1e3 / a.guide_frames,
-         {"workload": f"occluded-lights terrain (1M tris, 8 upward emitters capping chimneys light ceiling "
-                      f"spots, terrain lit indirectly), {w}x{h}, {spf} spp/frame, max depth "
-                      f"{maxdepth}, RR after {rr}; reference {a.guide_ref_frames * spf} spp plain "
-                      f"(GPU)",
-          "plain": {"seconds": round(tp, 3), "spp": a.guide_frames * spf, "relmse": err_p,
-                    "curve": cp},
-          "guided": {"seconds": round(tg, 3), "relmse": err_g, "reached": reached,
-                     "curve": cg, "adam_steps": int(info[2])},
-          "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
-                    "warm_frames": a.guide_warm_frames, "warm_steps_per_frame": a.guide_warm_steps,
-                    "records_per_step": a.guide_chunk,
-                    "lr_grid": cfg[3], "lr_mlp": a.guide_lr_mlp}},
-         higher=False)
     scene.close()
+    return {"ratio": round(ratio, 4) if reached else None,
+            "workload": f"occluded-lights terrain (1M tris, 8 upward emitters capping chimneys "
+                        f"light ceiling spots, terrain lit indirectly), {w}x{h}, {spf} spp/frame, "
+                        f"max depth {maxdepth}, RR after {rr}; reference "
+                        f"{a.guide_ref_frames * spf} spp plain (GPU)",
+            "plain": {"seconds": round(tp, 3), "frames": a.guide_frames,
+                      "spp": a.guide_frames * spf, "relmse": err_p, "curve": cp},
+            "guided": {"seconds": round(tg, 3), "frames": cg[-1][1] // spf if cg else None,
+                       "spp": cg[-1][1] if cg else None, "relmse": err_g, "reached": reached,
+                       "curve": cg, "adam_steps": int(info[2])},
+            "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
+                      "warm_frames": a.guide_warm_frames,
+                      "warm_steps_per_frame": a.guide_warm_steps,
+                      "records_per_step": a.guide_chunk,
+                      "lr_grid": cfg[3], "lr_mlp": a.guide_lr_mlp}}
+
+
+def bench_guide(a):
+    res = guide_equal_error(a)
+    if a.guide_profile_frames:
+        return 0
+    emit("guided/plain wall time to equal relMSE", res["ratio"] if res["ratio"] is not None
+         else float("nan"), "ratio", a.guide_frames, 0,
+         res["plain"]["seconds"] * 1e3 / a.guide_frames,
+         {k: v for k, v in res.items() if k != "ratio"}, higher=False)
     return 0
 
 
@@ -471,16 +493,24 @@ def main():
                     help="training records per Adam step")
     ap.add_argument("--guide-warm-steps", type=int, default=64)
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
+    ap.add_argument("--guide-profile-frames", type=int, default=0,
+                    help="--mode guide: only render this many guided frames (for profiling)")
+    ap.add_argument("--no-guide", action="store_true",
+                    help="trace mode: leave out the guided/plain equal-error run")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch path-tracing frames directly instead of replaying HIP graphs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path", file=sys.stderr)
         sys.exit(2)
+    L = lib()
+    if a.no_graphs:
+        check(L.rend_hip_pt_graphs(0), "graphs")
     if a.mode != "trace":
         sys.exit({"pt": bench_pt, "views": bench_views, "guide": bench_guide,
                   "tlas": bench_tlas}[a.mode](a))
 
-    L = lib()
     t0 = time.perf_counter()
     scene = Scene(config=2, seed=a.seed, detail=1.0, width=W, height=H, spp=1)
     log(f"scene: {scene.ntris} tris, build {time.perf_counter() - t0:.2f} s")
@@ -612,6 +642,16 @@ def main():
         "cpu_baseline": cpu,
     }
     scene.close()
+    if not a.no_guide:
+        # the path-guiding target on the same GPU in the same run: guided
+        # path tracing's wall time to plain's converged relMSE, over plain's
+        t = time.perf_counter()
+        try:
+            out["guided_vs_plain"] = guide_equal_error(a)
+        except Exception as e:  # the headline line is still printed
+            out["guided_vs_plain"] = {"error": f"{type(e).__name__}: {e}"}
+        log(f"guided vs plain: {time.perf_counter() - t:.0f} s, "
+            f"ratio {out['guided_vs_plain'].get('ratio')}")
     print(json.dumps(out), flush=True)
 
 

@@ -29,14 +29,20 @@ extern "C" {
 #define REND_ESTATE     -4   /* call order: scene not prepared/uploaded */
 #define REND_EOVERFLOW  -5   /* traversal stack or iteration limit hit */
 
-/* Reciprocal used by the ray/triangle test. */
-#define REND_RCP_HOST_TABLE  0   /* x86 RCPSS table: matches the reference bit for bit */
+/* Reciprocal used by the ray/triangle test (and the reciprocal square root
+ * of the reference-exact camera rays).  The reference's results depend on
+ * the CPU that runs it (Intel and AMD cores implement RCPSS/RSQRTPS
+ * differently); the library captures this host's instructions at set-up. */
+#define REND_RCP_HOST_TABLE  0   /* this host's RCPSS/RSQRTPS: the reference on this host, bit for bit */
 #define REND_RCP_IEEE        1   /* correctly rounded 1/det */
+#define REND_RCP_XEON_TABLE  2   /* the tables captured on the build host (an Intel Xeon):
+                                    reproduces results computed there */
 
 int          rend_hip_lasterror(void);
 const char  *rend_hip_errstr(void);
 
-/* Device properties (compute units, name) and the table's capture CPU. */
+/* Device properties (compute units, name) and the CPU whose tables the
+ * current mode uses (with the result of the set-up check of the rule). */
 int          rend_hip_devinfo(int *cus, char *name, unsigned int namelen);
 const char  *rend_hip_rcp_cpu(void);
 

@@ -34,6 +34,7 @@ REND_MODE_NORMALS = 0
 REND_MODE_PATHTRACE = 1
 REND_RCP_HOST_TABLE = 0
 REND_RCP_IEEE = 1
+REND_RCP_XEON_TABLE = 2
 REND_TLAS_HOST = 0
 REND_TLAS_GPU = 1
 INST_ID_BITS = 12

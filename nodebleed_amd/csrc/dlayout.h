@@ -59,14 +59,17 @@ struct CamArgs {
 };
 
 /* Traversal stack: the first DEPTH entries per lane are kept in LDS (DEPTH
- * depends on the kernel variant, REND_LDS_STACK_MIN..16), deeper ones spill
- * to a global buffer of (REND_STACK_MAX - DEPTH) entries per thread. */
+ * depends on the kernel variant, REND_LDS_STACK_MIN..RK_DEPTH0), deeper ones
+ * spill to a global buffer of (REND_STACK_MAX - DEPTH) entries per thread. */
 #define REND_STACK_MAX      64   /* reference bound, rend.c:834 assert */
-#define REND_LDS_STACK_MIN  8
+#define REND_LDS_STACK_MIN  5
+#define RK_DEPTH0           15   /* variant 0 (4 waves per SIMD) */
 #define RK_TRACE_VARIANTS   4
 /* rk_trace_launch flags: replace exactly-zero instance-space direction
    components by a signed 1e-20 (path-tracing callers only) */
 #define RK_TRACE_NUDGE      1u
+/* read the build host's (Intel Xeon) RCPSS table instead of this host's */
+#define RK_TRACE_XEON       2u
 /* bits 8..15 of the flags: lanes a leaf or setup step waits for (0: the
    default RK_STEP_BATCH) */
 #define RK_STEP_BATCH       8u
@@ -85,10 +88,10 @@ struct CamArgs {
 
 /* Allocation-free closest-hit launches (rend_hip_trace_noalloc) spill into
  * a static device array sized for at most RK_NOALLOC_BLOCKS persistent
- * blocks of the 16-entry LDS stack variant: 512 x 256 threads x 48 entries
- * x 8 bytes = 48 MiB per device. */
+ * blocks of the 15-entry LDS stack variant: 512 x 256 threads x 49 entries
+ * x 8 bytes = 49 MiB per device. */
 #define RK_NOALLOC_BLOCKS    512u
-#define RK_NOALLOC_DEPTH     16
+#define RK_NOALLOC_DEPTH     RK_DEPTH0
 #define RK_NOALLOC_MAX_RAYS  0x7fffffffu
 
 /* Error bits written by kernels into the device status word. */

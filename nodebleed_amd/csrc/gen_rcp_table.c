@@ -1,13 +1,15 @@
 /*
- * Captures the host CPU's approximate reciprocal (x86 RCPSS) as a 2048-entry
+ * Captures the host CPU's approximate reciprocal (x86 RCPSS) as a 4096-entry
  * mantissa table and proves, over all 2^32 single-precision inputs, that
  *
- *   rcp(x) = sign | (253 - E) << 23 | T[m >> 12]      for 1 <= E <= 252
+ *   rcp(x) = sign | (253 - E) << 23 | T[m >> 11]      for 1 <= E <= 252
  *          = sign | 0x7f800000                       for E == 0 (zero, denormal)
  *          = sign (signed zero)                      for 253 <= E <= 254, or +-inf
  *          = x | 0x00400000                          for NaN (quieted)
  *
- * where E is the biased exponent and m the 23-bit mantissa of x.
+ * where E is the biased exponent and m the 23-bit mantissa of x.  Intel
+ * cores read 11 leading mantissa bits (pairs of equal entries), AMD Zen
+ * cores 12; both satisfy the rule.
  *
  * The reference's Moeller-Trumbore test computes idet with _mm_rcp_ps
  * (rend.c:872, rend.c:1930), so bit-exact hit t/u/v need this exact table on
@@ -37,14 +39,14 @@ static uint32_t table_rcp(const uint32_t *tab, uint32_t u)
 		return m ? (u | 0x00400000u) : s;
 	if (e >= 253)
 		return s;
-	return s | ((253u - e) << 23) | tab[m >> 12];
+	return s | ((253u - e) << 23) | tab[m >> 11];
 }
 
 int main(void)
 {
-	static uint32_t tab[2048];
-	for (uint32_t i = 0; i < 2048; i++) {
-		uint32_t r = rcpss_bits(0x3f800000u | (i << 12));
+	static uint32_t tab[4096];
+	for (uint32_t i = 0; i < 4096; i++) {
+		uint32_t r = rcpss_bits(0x3f800000u | (i << 11));
 		if (((r >> 23) & 0xffu) != 126u) {
 			fprintf(stderr, "unexpected exponent at %u: %08x\n", i, r);
 			return 1;
@@ -94,13 +96,13 @@ int main(void)
 	  " * (rule in gen_rcp_table.c).  Do not edit. */\n", model);
 	printf("#ifndef RCP_TABLE_H\n#define RCP_TABLE_H\n");
 	printf("#define RCP_TABLE_CPU \"%s\"\n", model);
-	printf("/* Result mantissa of RCPSS for inputs 1 + i/2048 (exponent 126). */\n");
+	printf("/* Result mantissa of RCPSS for inputs 1 + i/4096 (exponent 126). */\n");
 	printf("#define RCP_TABLE_INIT { \\\n");
-	for (int i = 0; i < 2048; i++)
+	for (int i = 0; i < 4096; i++)
 		printf("0x%06x,%s", tab[i], (i % 8 == 7) ? " \\\n" : " ");
 	printf("}\n");
 	printf("#ifndef RCP_TABLE_NO_HOST_ARRAY\n"
-	  "static const unsigned int rcp_mant_table[2048] = RCP_TABLE_INIT;\n"
+	  "static const unsigned int rcp_mant_table[4096] = RCP_TABLE_INIT;\n"
 	  "#endif\n#endif\n");
 	return 0;
 }

@@ -1,9 +1,9 @@
 /*
  * Captures the host CPU's approximate reciprocal square root (x86 RSQRTPS)
- * as two 2048-entry tables (odd and even biased exponent) and proves, over
+ * as two 4096-entry tables (odd and even biased exponent) and proves, over
  * all 2^32 single-precision inputs, that
  *
- *   rsqrt(x) = T[p][m >> 12] - (((E - B_p) / 2) << 23)   x > 0 normal,
+ *   rsqrt(x) = T[p][m >> 11] - (((E - B_p) / 2) << 23)   x > 0 normal,
  *                                                        p = E & 1, B_1 = 127,
  *                                                        B_0 = 128
  *            = sign | 0x7f800000    zero or denormal (the sign of x kept)
@@ -12,7 +12,8 @@
  *            = x | 0x00400000       NaN (quieted)
  *
  * where E is the biased exponent, m the 23-bit mantissa and T[p][i] the
- * instruction's result for the input with exponent B_p and mantissa i << 12.
+ * instruction's result for the input with exponent B_p and mantissa i << 11
+ * (Intel cores read 11 leading mantissa bits, AMD Zen cores 12).
  *
  * The reference normalises camera-ray directions with _mm256_rsqrt_ps
  * (rend.c:3370, rend.c:3397), so bit-exact primary rays need this exact
@@ -45,15 +46,15 @@ static uint32_t table_rsqrt(const uint32_t *tab, uint32_t u)
 	uint32_t p = e & 1u;
 	int b = p ? 127 : 128;
 	int sh = ((int)e - b) / 2;  /* exact: e - b is even */
-	return (uint32_t)((int32_t)tab[(p << 11) | (m >> 12)] - sh * (1 << 23));
+	return (uint32_t)((int32_t)tab[(p << 12) | (m >> 11)] - sh * (1 << 23));
 }
 
 int main(void)
 {
-	static uint32_t tab[4096];
+	static uint32_t tab[8192];
 	for (uint32_t p = 0; p < 2; p++)
-		for (uint32_t i = 0; i < 2048; i++)
-			tab[(p << 11) | i] = rsqrtss_bits(((p ? 127u : 128u) << 23) | (i << 12));
+		for (uint32_t i = 0; i < 4096; i++)
+			tab[(p << 12) | i] = rsqrtss_bits(((p ? 127u : 128u) << 23) | (i << 11));
 
 	/* Exhaustive check, 8 lanes at a time. */
 	uint64_t bad = 0;
@@ -97,9 +98,9 @@ int main(void)
 	       "#ifndef RSQRT_TABLE_H\n#define RSQRT_TABLE_H\n"
 	       "#define RSQRT_TABLE_CPU \"%s\"\n"
 	       "/* [p][i]: result for the input with biased exponent 127 (p = 1) or\n"
-	       " * 128 (p = 0) and mantissa i << 12; index (p << 11) | i. */\n"
+	       " * 128 (p = 0) and mantissa i << 11; index (p << 12) | i. */\n"
 	       "#define RSQRT_TABLE_INIT { \\\n", model, model);
-	for (int i = 0; i < 4096; i++)
+	for (int i = 0; i < 8192; i++)
 		printf("0x%08x,%s", tab[i], (i % 8 == 7) ? " \\\n" : " ");
 	printf("}\n#endif\n");
 	return 0;

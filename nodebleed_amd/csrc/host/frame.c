@@ -296,22 +296,26 @@ int rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays, unsigned int w,
 	return r ? rl_hipfail(r, "camera rays") : 0;
 }
 
-static const unsigned g_rsqrt_tab[4096] = RSQRT_TABLE_INIT;
+static const unsigned g_rsqrt_xeon[8192] = RSQRT_TABLE_INIT;
 static int g_rsqrt_dev = -1;  /* device the table was uploaded to */
+static int g_rsqrt_mode = -1; /* 0: this host's capture, 1: the build host's */
 
-/* Host RSQRTPS table in the camera kernel's constant memory (once per
- * device). */
+/* RSQRTPS table of the selected host in the camera kernel's constant memory:
+ * this host's (captured at set-up) unless the build host's tables are
+ * selected (REND_RCP_XEON_TABLE). */
 static int rsqrt_table_ready(hipStream_t st)
 {
 	int dev = 0;
 	hipError_t e = hipGetDevice(&dev);
 	if (e)
 		return rl_hipfail(e, "device");
-	if (g_rsqrt_dev != dev) {
-		int r = rk_camrays_ref_setup(g_rsqrt_tab, st);
+	const int mode = G.rcp_mode == REND_RCP_XEON_TABLE;
+	if (g_rsqrt_dev != dev || g_rsqrt_mode != mode) {
+		int r = rk_camrays_ref_setup(mode ? g_rsqrt_xeon : G.rsqrt_host, st);
 		if (r)
 			return rl_hipfail(r, "rsqrt table");
 		g_rsqrt_dev = dev;
+		g_rsqrt_mode = mode;
 	}
 	return 0;
 }

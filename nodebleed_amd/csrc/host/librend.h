@@ -6,6 +6,8 @@
 #define LIBREND_H
 
 #include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
 
 #include "rend.h"
 #include "../dlayout.h"
@@ -18,6 +20,11 @@ struct rl_global {
 	int                 cus;
 	char                name[256];
 	int                 rcp_ieee;
+	int                 rcp_mode;     /* REND_RCP_HOST_TABLE / _IEEE / _XEON_TABLE */
+	uint16_t            rcp_host[4096];   /* this host's RCPSS table (approx.c) */
+	uint32_t            rsqrt_host[8192]; /* this host's RSQRTPS tables */
+	unsigned long long  approx_bad;   /* inputs that broke the table rule */
+	char                approx_cpu[192];
 	unsigned            refill;      /* idle lanes that trigger a wave refill */
 	unsigned            max_iter;    /* traversal step limit per ray */
 	unsigned            grid_mult;   /* persistent blocks per CU, 0 = occupancy */
@@ -116,6 +123,7 @@ int              rl_trace_async(struct rl_scene *s, const void *d_rays,
                                 int closest, int count, hipStream_t st);
 int              rl_check_errors(hipStream_t st);
 int              rl_stream_order(hipStream_t st);
+void             rl_approx_ready(void);
 unsigned         rl_trace_flags(unsigned extra);
 int              rl_stream_mark(hipStream_t st);
 unsigned         rl_blkszx(void);
@@ -143,13 +151,19 @@ void             rl_camargs(const struct rcam *c, struct CamArgs *a);
 int rk_trace_variants(void);
 int rk_trace_lds_depth(int variant);
 int rk_trace_occupancy(int closest, int count, int rcp_ieee, int variant);
+unsigned long long rl_approx_capture(uint16_t rcp[4096], uint32_t rsq[8192], char *cpu,
+                                     size_t cpulen);
+uint32_t rl_rcp_rule(const uint16_t *tab, uint32_t u);
+uint32_t rl_rsqrt_rule(const uint32_t *tab, uint32_t u);
+int rk_trace_set_host_rcp(const uint16_t *tab);
 int rk_scratch_init(void);
 unsigned long long rk_scratch_bytes(void);
 int rk_trace_launch_noalloc(const void *arena, unsigned long long tlas,
                             const void *insts, const void *rays, void *hits,
                             unsigned int n, unsigned int *ctl, int rcp_ieee,
                             unsigned int cus, unsigned int refill,
-                            unsigned int max_iter, unsigned int *diag, void *stream);
+                            unsigned int max_iter, unsigned int *diag, unsigned int flags,
+                            void *stream);
 int rk_trace_launch(const void *arena, unsigned long long tlas,
                     const void *insts, const void *rays, void *hits,
                     void *occ, unsigned int n, unsigned int *ctl,

@@ -49,7 +49,32 @@ int rl_hipfail(int herr, const char *what)
 
 int rend_hip_lasterror(void) { return g_err; }
 const char *rend_hip_errstr(void) { return g_errmsg; }
-const char *rend_hip_rcp_cpu(void) { return RCP_TABLE_CPU; }
+/* This host's RCPSS/RSQRTPS tables, captured once (approx.c). */
+static pthread_once_t g_approx_once = PTHREAD_ONCE_INIT;
+
+static void approx_capture(void)
+{
+	G.approx_bad = rl_approx_capture(G.rcp_host, G.rsqrt_host, G.approx_cpu,
+	  sizeof(G.approx_cpu));
+}
+
+void rl_approx_ready(void)
+{
+	pthread_once(&g_approx_once, approx_capture);
+}
+
+const char *rend_hip_rcp_cpu(void)
+{
+	static char desc[256];
+	if (G.rcp_mode == REND_RCP_XEON_TABLE)
+		return RCP_TABLE_CPU;
+	if (G.rcp_mode == REND_RCP_IEEE)
+		return "IEEE (correctly rounded)";
+	rl_approx_ready();
+	snprintf(desc, sizeof(desc), "%s%s", G.approx_cpu,
+	  G.approx_bad ? " (table rule failed: not bit-exact)" : "");
+	return desc;
+}
 
 /* ---- process-global device state ----------------------------------------- */
 
@@ -79,6 +104,12 @@ int rl_device(void)
 	      != hipSuccess) {
 		pthread_mutex_unlock(&g_lock);
 		return rl_hipfail(e, "device init");
+	}
+	/* this host's RCPSS/RSQRTPS for the reference-exact kernels */
+	rl_approx_ready();
+	if ((e = (hipError_t)rk_trace_set_host_rcp(G.rcp_host)) != hipSuccess) {
+		pthread_mutex_unlock(&g_lock);
+		return rl_hipfail(e, "host RCPSS table");
 	}
 	G.cus = p.multiProcessorCount;
 	snprintf(G.name, sizeof(G.name), "%s", p.gcnArchName);
@@ -121,9 +152,10 @@ int rend_hip_devinfo(int *cus, char *name, unsigned int namelen)
 
 int rend_hip_set_rcp(int mode)
 {
-	if (mode != REND_RCP_HOST_TABLE && mode != REND_RCP_IEEE)
+	if (mode != REND_RCP_HOST_TABLE && mode != REND_RCP_IEEE && mode != REND_RCP_XEON_TABLE)
 		return rl_fail(REND_EINVAL, "rcp mode %d", mode);
 	G.rcp_ieee = mode == REND_RCP_IEEE;
+	G.rcp_mode = mode;
 	return 0;
 }
 
@@ -169,7 +201,8 @@ int rend_hip_set_step_batch(unsigned int lanes)
 /* Launch flags of every traversal launch: bits 8..15 carry the step batch. */
 unsigned rl_trace_flags(unsigned extra)
 {
-	return extra | (G.step_batch << 8);
+	return extra | (G.step_batch << 8) |
+	  (G.rcp_mode == REND_RCP_XEON_TABLE ? RK_TRACE_XEON : 0u);
 }
 
 int rend_hip_get_tuning(unsigned int out[3])
@@ -1023,7 +1056,7 @@ int rend_hip_trace_noalloc(struct rdata *rd, const struct rray *d_rays,
 	unsigned slot = G.ctl_next++ % RL_CTL_SLOTS;
 	int e = rk_trace_launch_noalloc(s->arena, s->tlas, s->insts, d_rays, d_hits, n,
 	  G.ctl + RK_CTL_WORDS * slot, G.rcp_ieee, (unsigned)G.cus, G.refill, G.max_iter,
-	  G.diag, st);
+	  G.diag, rl_trace_flags(0), st);
 	if (e)
 		return rl_hipfail(e, "trace_noalloc launch");
 	G.last_slot = slot;

@@ -259,7 +259,10 @@ __global__ void k_fill_f32(float *p, size_t n, float v) {
 // ray is bit-identical.  One thread per packet replays its block's
 // generator 2 k steps ahead.
 
-__constant__ uint32_t c_rsqrt_tab[4096];
+// RSQRTPS results for the 12 leading mantissa bits at biased exponents 128
+// (index 0..4095) and 127 (4096..8191): this host's or the build host's
+// capture, as the library uploads them.
+__constant__ uint32_t c_rsqrt_tab[8192];
 
 struct Xs128 {
   uint64_t s0, s1;
@@ -293,7 +296,7 @@ __device__ __forceinline__ Xs128 xs128_jump(Xs128 in) {
 }
 
 // x86 RSQRTPS from the captured tables (rule proven over all inputs by
-// gen_rsqrt_table.c).
+// gen_rsqrt_table.c on Intel and AMD cores).
 __device__ __forceinline__ float rsqrt_host(float x) {
   const uint32_t u = __float_as_uint(x);
   const uint32_t s = u & 0x80000000u, e = (u >> 23) & 0xffu, m = u & 0x7fffffu;
@@ -304,7 +307,7 @@ __device__ __forceinline__ float rsqrt_host(float x) {
   else {
     const uint32_t p = e & 1u;
     const int sh = ((int)e - (p ? 127 : 128)) / 2;
-    r = (uint32_t)((int32_t)c_rsqrt_tab[(p << 11) | (m >> 12)] - sh * (1 << 23));
+    r = (uint32_t)((int32_t)c_rsqrt_tab[(p << 12) | (m >> 11)] - sh * (1 << 23));
   }
   return __uint_as_float(r);
 }
@@ -389,7 +392,7 @@ __global__ void k_shade_normals_ref(const rhit *__restrict__ hits,
 }
 
 extern "C" int rk_camrays_ref_setup(const unsigned *tab, void *stream) {
-  return (int)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rsqrt_tab), tab, 4096 * sizeof(uint32_t), 0,
+  return (int)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rsqrt_tab), tab, 8192 * sizeof(uint32_t), 0,
                                      hipMemcpyHostToDevice, (hipStream_t)stream);
 }
 

@@ -33,7 +33,12 @@
 
 #define FLT_MAXF 3.402823466e+38f
 
-__constant__ uint32_t c_rcp_tab[2048] = RCP_TABLE_INIT;
+// RCPSS mantissa tables (12 leading input bits): the one captured on the
+// build host (an Intel Xeon, for results reproduced from fixtures made
+// there) and this host's, captured by the library at set-up
+// (rk_trace_set_host_rcp); RK_TRACE_XEON selects the former.
+__constant__ uint32_t c_rcp_xeon[4096] = RCP_TABLE_INIT;
+__constant__ uint16_t c_rcp_host[4096];
 
 namespace {
 
@@ -74,16 +79,16 @@ __device__ __forceinline__ float nudge0(float x) {
   return x == 0.0f ? __builtin_copysignf(1e-20f, x) : x;
 }
 
-// x86 RCPSS from the captured mantissa table (rule proven exhaustively by
-// gen_rcp_table.c).
-// Table entries keep 12 significant bits (the low 11 mantissa bits of every
-// captured result are zero), so LDS holds them as 16-bit values.
+// x86 RCPSS from a captured mantissa table indexed by the 12 leading
+// mantissa bits (rule proven exhaustively by gen_rcp_table.c on Intel and
+// AMD cores).  Entries keep 12 significant bits (the low 11 mantissa bits of
+// every result are zero), so LDS holds them as 16-bit values.
 __device__ __forceinline__ float rcp_table(float x, const uint16_t *tab) {
   uint32_t u = __float_as_uint(x);
   uint32_t s = u & 0x80000000u;
   uint32_t e = (u >> 23) & 0xffu;
   uint32_t m = u & 0x7fffffu;
-  uint32_t r = s | ((253u - e) << 23) | ((uint32_t)tab[m >> 12] << 11);
+  uint32_t r = s | ((253u - e) << 23) | ((uint32_t)tab[m >> 11] << 11);
   r = (e >= 253u) ? s : r;
   r = (e == 255u && m != 0u) ? (u | 0x00400000u) : r;
   r = (e == 0u) ? (s | 0x7f800000u) : r;
@@ -161,7 +166,7 @@ __device__ __forceinline__ uint32_t slab8(const dnode *ndp, const RayS &cr, floa
 
 // DEPTH: LDS stack entries per lane; MINW: minimum waves per SIMD the
 // register allocation must allow (occupancy variants; LDS per block is
-// 4 KB table + DEPTH * 2 KB).
+// 8 KB table + (DEPTH + 1) * 2 KB).
 template <bool CLOSEST, bool COUNT, bool RCP_IEEE, int DEPTH, int MINW>
 __global__ __launch_bounds__(REND_TRACE_BLOCK, MINW)
 void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
@@ -171,14 +176,15 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
              unsigned long long *__restrict__ stats, uint32_t refill,
              uint32_t max_iter, const uint32_t *__restrict__ n_dev,
              uint32_t *__restrict__ diag, uint32_t flags) {
-  __shared__ uint16_t s_tab[2048];
+  __shared__ uint16_t s_tab[RCP_IEEE ? 2 : 4096];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
   // stack rows [0, DEPTH) per lane, plus one junk row that branch-free
   // pushes write for children they do not keep
   __shared__ uint2 s_stack[DEPTH + 1][REND_TRACE_BLOCK];
   if (!RCP_IEEE) {
-    for (uint32_t i = threadIdx.x; i < 2048; i += REND_TRACE_BLOCK)
-      s_tab[i] = (uint16_t)(c_rcp_tab[i] >> 11);
+    const bool xeon = (flags & RK_TRACE_XEON) != 0u;
+    for (uint32_t i = threadIdx.x; i < 4096; i += REND_TRACE_BLOCK)
+      s_tab[i] = xeon ? (uint16_t)(c_rcp_xeon[i] >> 11) : c_rcp_host[i];
     __syncthreads();
   }
 
@@ -582,9 +588,9 @@ int occ_one() {
 }
 
 // Occupancy variants (LDS stack depth, waves per SIMD); LDS per block is
-// 4 KB + depth * 2 KB, so 160 KB of LDS holds 4, 5, 6 and 8 blocks.
+// 8 KB + (depth + 1) * 2 KB, so 160 KB of LDS holds 4, 5, 6 and 8 blocks.
 // The counting kernel exists for variant 0 only.
-#define RK_VARIANTS(X) X(0, 16, 4) X(1, 12, 5) X(2, 10, 6) X(3, 8, 8)
+#define RK_VARIANTS(X) X(0, 15, 4) X(1, 11, 5) X(2, 8, 6) X(3, 5, 8)
 
 template <bool C, bool I>
 hipError_t launch_v(int v, uint32_t grid, hipStream_t s, const LaunchArgs &a) {
@@ -606,14 +612,20 @@ int occ_v(int v) {
 // C-ABI between the host library and the traversal kernels.
 extern "C" int rk_trace_variants(void) { return RK_TRACE_VARIANTS; }
 
+// This host's RCPSS table (12-bit entries, index = 12 leading mantissa bits).
+extern "C" int rk_trace_set_host_rcp(const uint16_t *tab) {
+  return (int)hipMemcpyToSymbol(HIP_SYMBOL(c_rcp_host), tab, 4096 * sizeof(uint16_t), 0,
+                                hipMemcpyHostToDevice);
+}
+
 extern "C" int rk_trace_occupancy(int closest, int count, int rcp_ieee, int variant) {
   if (count) {
     const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
     switch (k) {
-      case 0: return occ_one<false, true, false, 16, 4>();
-      case 1: return occ_one<false, true, true, 16, 4>();
-      case 2: return occ_one<true, true, false, 16, 4>();
-      default: return occ_one<true, true, true, 16, 4>();
+      case 0: return occ_one<false, true, false, RK_DEPTH0, 4>();
+      case 1: return occ_one<false, true, true, RK_DEPTH0, 4>();
+      case 2: return occ_one<true, true, false, RK_DEPTH0, 4>();
+      default: return occ_one<true, true, true, RK_DEPTH0, 4>();
     }
   }
   const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
@@ -668,10 +680,10 @@ extern "C" int rk_trace_launch(const void *arena, unsigned long long tlas,
   const int k = (closest ? 2 : 0) | (rcp_ieee ? 1 : 0);
   if (count) {
     switch (k) {
-      case 0: e = launch_one<false, true, false, 16, 4>(grid, s, a); break;
-      case 1: e = launch_one<false, true, true, 16, 4>(grid, s, a); break;
-      case 2: e = launch_one<true, true, false, 16, 4>(grid, s, a); break;
-      default: e = launch_one<true, true, true, 16, 4>(grid, s, a); break;
+      case 0: e = launch_one<false, true, false, RK_DEPTH0, 4>(grid, s, a); break;
+      case 1: e = launch_one<false, true, true, RK_DEPTH0, 4>(grid, s, a); break;
+      case 2: e = launch_one<true, true, false, RK_DEPTH0, 4>(grid, s, a); break;
+      default: e = launch_one<true, true, true, RK_DEPTH0, 4>(grid, s, a); break;
     }
     return (int)e;
   }
@@ -722,7 +734,7 @@ extern "C" int rk_trace_launch_noalloc(const void *arena, unsigned long long tla
                                        unsigned int n, unsigned int *ctl, int rcp_ieee,
                                        unsigned int cus, unsigned int refill,
                                        unsigned int max_iter, unsigned int *diag,
-                                       void *stream) {
+                                       unsigned int flags, void *stream) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return (int)e;
@@ -751,7 +763,7 @@ extern "C" int rk_trace_launch_noalloc(const void *arena, unsigned long long tla
   a.max_iter = max_iter;
   a.n_dev = nullptr;
   a.diag = diag;
-  a.flags = 0;
+  a.flags = flags & ~RK_TRACE_NUDGE;
   e = rcp_ieee ? launch_one<true, false, true, RK_NOALLOC_DEPTH, 4>(grid, s, a)
                : launch_one<true, false, false, RK_NOALLOC_DEPTH, 4>(grid, s, a);
   return (int)e;

@@ -6,8 +6,9 @@
  *   4x2-pixel packets in Morton order              rend.c:131-162, util.c:133-145
  *   rand8_init / xorshift128+ / jump / randf8      util.c:29-75, 99-107
  *   make_camray_pckt, pinhole branch               rend.c:3293-3317, 3377-3401
- *   _mm256_rsqrt_ps                                host RSQRTPS table
- *                                                  (gen_rsqrt_table.c rule)
+ *   _mm256_rsqrt_ps                                this host's RSQRTPS (mode 0)
+ *                                                  or the build host's table
+ *                                                  (mode 2, gen_rsqrt_table.c rule)
  * Fused operations of the reference (fmsub/fmadd) are fmaf here; everything
  * else is separately rounded (-ffp-contract=off).
  * Pinned by: the reference's own make_camray_pckt compiled into
@@ -23,7 +24,13 @@
 #include "rend_rays.h"
 #include "../nodebleed_amd/csrc/rsqrt_table.h"
 
-static const uint32_t rsq_tab[4096] = RSQRT_TABLE_INIT;
+static const uint32_t rsq_tab[8192] = RSQRT_TABLE_INIT;
+static int g_rsq_mode;  /* 0 host instruction, 2 build host's table */
+
+void or_camrays_mode(int mode)
+{
+	g_rsq_mode = mode;
+}
 
 static float f32(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static uint32_t u32(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -40,7 +47,14 @@ float or_rsqrt(float x)
 		return f32(0xffc00000u);
 	const uint32_t p = e & 1u;
 	const int sh = ((int)e - (p ? 127 : 128)) / 2;
-	return f32((uint32_t)((int32_t)rsq_tab[(p << 11) | (m >> 12)] - sh * (1 << 23)));
+	return f32((uint32_t)((int32_t)rsq_tab[(p << 12) | (m >> 11)] - sh * (1 << 23)));
+}
+
+static float cam_rsqrt(float x)
+{
+	if (g_rsq_mode == 2)
+		return or_rsqrt(x);
+	return _mm_cvtss_f32(_mm_rsqrt_ss(_mm_set_ss(x)));
 }
 
 struct xs { uint64_t s0, s1; };
@@ -121,7 +135,7 @@ void or_camrays(const struct rdata *rd, unsigned bszx, unsigned bszy, unsigned s
 			float t[3];
 			for (int a = 0; a < 3; a++)
 				t[a] = fmaf(fw[a], fd, fmaf(ri[a], sx, -(up[a] * sy)));
-			const float rl = or_rsqrt(fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2])));
+			const float rl = cam_rsqrt(fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2])));
 			struct rray *r = &out[k * 8 + (unsigned)l];
 			r->ox = eye[0];
 			r->oy = eye[1];

@@ -12,8 +12,11 @@
  * It keeps the reference's eager stack compaction and perm8-ordered pushes
  * (the GPU kernel uses lazy culling and an inverse permutation instead), the
  * x86 MAXPS/MINPS operand rules, the FMA shapes of the intrinsics and the
- * host RCPSS table (exhaustively verified against the instruction by
- * nodebleed_amd/csrc/gen_rcp_table.c).  Compile with -ffp-contract=off.
+ * reference's reciprocal: this host's RCPSS instruction itself (mode 0, the
+ * reference on this host), the correctly rounded 1/det (mode 1), or the
+ * build host's RCPSS restated from its table (mode 2; the table is verified
+ * against the instruction for all inputs by gen_rcp_table.c on that host).
+ * Compile with -ffp-contract=off.
  *
  * Visit counters (interior nodes, leaf4 records, instance transforms) give
  * the algorithmic traffic of each ray.  Batches run on N threads claiming
@@ -21,6 +24,7 @@
  * (rend.c:3623).
  */
 #include <float.h>
+#include <immintrin.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -31,7 +35,7 @@
 #define RCP_TABLE_NO_HOST_ARRAY
 #include "../nodebleed_amd/csrc/rcp_table.h"
 
-static const unsigned int rcp_tab[2048] = RCP_TABLE_INIT;
+static const unsigned int rcp_tab[4096] = RCP_TABLE_INIT;
 
 #define LEAF 0x80000000u
 #define F(v, i) (((const float *)&(v))[i])
@@ -41,16 +45,14 @@ struct ocnt {
 	unsigned long long nodes, leaves, insts;
 };
 
-static int g_rcp_ieee;
+static int g_rcp_mode;  /* 0 host instruction, 1 IEEE, 2 build host's table */
 
 static float bits2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 static uint32_t f2bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
 
-/* _mm_rcp_ps on the capture host (rule in gen_rcp_table.c). */
+/* _mm_rcp_ps on the build host (rule in gen_rcp_table.c). */
 float or_rcp(float x)
 {
-	if (g_rcp_ieee)
-		return 1.0f / x;
 	uint32_t u = f2bits(x), s = u & 0x80000000u, e = (u >> 23) & 0xffu,
 	  m = u & 0x7fffffu;
 	if (e == 0)
@@ -59,7 +61,17 @@ float or_rcp(float x)
 		return bits2f(m ? (u | 0x00400000u) : s);
 	if (e >= 253)
 		return bits2f(s);
-	return bits2f(s | ((253u - e) << 23) | rcp_tab[m >> 12]);
+	return bits2f(s | ((253u - e) << 23) | rcp_tab[m >> 11]);
+}
+
+/* The reciprocal of the triangle test in the current mode. */
+static float det_rcp(float x)
+{
+	if (g_rcp_mode == 1)
+		return 1.0f / x;
+	if (g_rcp_mode == 2)
+		return or_rcp(x);
+	return _mm_cvtss_f32(_mm_rcp_ss(_mm_set_ss(x)));
 }
 
 /* MAXPS/MINPS: second operand unless the first compares greater/less. */
@@ -127,7 +139,7 @@ static unsigned mt4(const struct leaf4 *l, const struct ray *r, float tcur,
 		float qvx = fmaf(tvy, e0z, -(tvz * e0y));
 		float qvy = fmaf(tvz, e0x, -(tvx * e0z));
 		float qvz = fmaf(tvx, e0y, -(tvy * e0x));
-		float idet = or_rcp(det);
+		float idet = det_rcp(det);
 		float u = idet * fmaf(tvx, pvx, fmaf(tvy, pvy, tvz * pvz));
 		float v = idet * fmaf(r->dx, qvx, fmaf(r->dy, qvy, r->dz * qvz));
 		float t = idet * fmaf(e1x, qvx, fmaf(e1y, qvy, e1z * qvz));
@@ -471,7 +483,7 @@ static int run(struct job *j, int threads, unsigned long long cnt[3])
  * cnt (optional) receives total node/leaf/instance visits; per_ray (optional)
  * receives them per ray. */
 int or_trace(const struct rdata *rd, const struct rray *rays,
-             struct rhit *hits, unsigned n, int rcp_ieee, int threads,
+             struct rhit *hits, unsigned n, int rcp_mode, int threads,
              unsigned long long cnt[3], unsigned long long *per_ray)
 {
 	struct job j;
@@ -481,12 +493,12 @@ int or_trace(const struct rdata *rd, const struct rray *rays,
 	j.hits = hits;
 	j.n = n;
 	j.per_ray = per_ray;
-	g_rcp_ieee = rcp_ieee;
+	g_rcp_mode = rcp_mode;
 	return run(&j, threads, cnt);
 }
 
 int or_occluded(const struct rdata *rd, const struct rray *rays,
-                unsigned char *occ, unsigned n, int rcp_ieee, int threads)
+                unsigned char *occ, unsigned n, int rcp_mode, int threads)
 {
 	struct job j;
 	memset(&j, 0, sizeof(j));
@@ -494,16 +506,16 @@ int or_occluded(const struct rdata *rd, const struct rray *rays,
 	j.rays = rays;
 	j.occ = occ;
 	j.n = n;
-	g_rcp_ieee = rcp_ieee;
+	g_rcp_mode = rcp_mode;
 	return run(&j, threads, NULL);
 }
 
 /* Brute force over every triangle of every instance: the closest t under the
  * same Moeller-Trumbore arithmetic (no BVH), for small scenes. */
 void or_bruteforce(const struct rdata *rd, const struct rray *rays,
-                   struct rhit *hits, unsigned n, int rcp_ieee)
+                   struct rhit *hits, unsigned n, int rcp_mode)
 {
-	g_rcp_ieee = rcp_ieee;
+	g_rcp_mode = rcp_mode;
 	for (unsigned i = 0; i < n; i++) {
 		const struct rray *in = &rays[i];
 		float best = in->tmax, bu = 0, bv = 0;

@@ -47,40 +47,52 @@ def oracle() -> C.CDLL:
             getattr(L, f).argtypes = [C.c_float, C.c_float]
         L.or_rsqrt.restype = C.c_float
         L.or_rsqrt.argtypes = [C.c_float]
+        L.or_camrays_mode.restype = None
+        L.or_camrays_mode.argtypes = [C.c_int]
         L.or_camrays.restype = None
         L.or_camrays.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_void_p]
         _or = L
     return _or
 
 
+#: reciprocal modes of the oracle and of rend_hip_set_rcp: this host's
+#: RCPSS (the reference on this host), the correctly rounded 1/det, the
+#: build host's (Intel Xeon) table
+RCP_MODES = {"host": 0, "ieee": 1, "xeon": 2}
+
+
+def _rcp_mode(rcp_ieee, rcp):
+    return 1 if rcp_ieee else RCP_MODES[rcp]
+
+
 def or_trace(scene: Scene, rays: np.ndarray, rcp_ieee=False, threads=8,
-             per_ray=False):
+             per_ray=False, rcp="host"):
     rays = np.ascontiguousarray(rays, dtype=RAY)
     hits = np.zeros(len(rays), HIT)
     cnt = (C.c_ulonglong * 3)()
     pr = np.zeros((len(rays), 3), np.uint64) if per_ray else None
     err = oracle().or_trace(C.c_void_p(scene.rd), ptr(rays), ptr(hits), len(rays),
-                            int(rcp_ieee), threads, cnt,
+                            _rcp_mode(rcp_ieee, rcp), threads, cnt,
                             ptr(pr) if per_ray else None)
     assert err == 0, f"oracle stack overflow on {err} rays"
     out = (hits, tuple(cnt))
     return out + (pr,) if per_ray else out
 
 
-def or_occluded(scene: Scene, rays: np.ndarray, rcp_ieee=False, threads=8):
+def or_occluded(scene: Scene, rays: np.ndarray, rcp_ieee=False, threads=8, rcp="host"):
     rays = np.ascontiguousarray(rays, dtype=RAY)
     occ = np.zeros(len(rays), np.uint8)
     err = oracle().or_occluded(C.c_void_p(scene.rd), ptr(rays), ptr(occ), len(rays),
-                               int(rcp_ieee), threads)
+                               _rcp_mode(rcp_ieee, rcp), threads)
     assert err == 0
     return occ
 
 
-def or_bruteforce(scene: Scene, rays: np.ndarray, rcp_ieee=False):
+def or_bruteforce(scene: Scene, rays: np.ndarray, rcp_ieee=False, rcp="host"):
     rays = np.ascontiguousarray(rays, dtype=RAY)
     hits = np.zeros(len(rays), HIT)
     oracle().or_bruteforce(C.c_void_p(scene.rd), ptr(rays), ptr(hits), len(rays),
-                           int(rcp_ieee))
+                           _rcp_mode(rcp_ieee, rcp))
     return hits
 
 
@@ -103,13 +115,20 @@ def rel_mse(x: np.ndarray, r: np.ndarray) -> float:
     return float(((x - r) ** 2).sum() / (r ** 2).sum())
 
 
-def or_camrays(scene: Scene, bszx: int, bszy: int, samples: int, blocks) -> np.ndarray:
+def or_camrays(scene: Scene, bszx: int, bszy: int, samples: int, blocks,
+               rsqrt: str = "host") -> np.ndarray:
     """The reference's primary rays of the given blocks (pinhole camera),
-    restated: rays of block b in rend_render's packet/lane order."""
+    restated: rays of block b in rend_render's packet/lane order, with this
+    host's RSQRTPS or the build host's (rsqrt="xeon")."""
     npk = bszx * bszy // 8
     out = np.zeros((len(blocks), npk * 8), RAY)
-    for i, b in enumerate(blocks):
-        oracle().or_camrays(C.c_void_p(scene.rd), bszx, bszy, samples, int(b), ptr(out[i]))
+    oracle().or_camrays_mode(RCP_MODES[rsqrt])
+    try:
+        for i, b in enumerate(blocks):
+            oracle().or_camrays(C.c_void_p(scene.rd), bszx, bszy, samples, int(b),
+                                ptr(out[i]))
+    finally:
+        oracle().or_camrays_mode(0)
     return out
 
 

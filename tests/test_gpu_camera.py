@@ -7,8 +7,8 @@ import sys
 import numpy as np
 import pytest
 
-from nodebleed_amd import RAY, Scene, check, lib
-from oracle_lib import or_camrays, or_trace
+from nodebleed_amd import REND_RCP_HOST_TABLE, REND_RCP_XEON_TABLE, RAY, Scene, check, lib
+from oracle_lib import or_camrays, or_trace, ref_camrays, reference
 from raygen import mismatches
 
 pytestmark = pytest.mark.gpu
@@ -63,4 +63,33 @@ def test_reference_camrays_reject_partial_packets():
     finally:
         L.rend_hip_free(d)
         L.rend_staticinit(64, 32)
+        s.close()
+
+
+@pytest.mark.parametrize("mode", ["host", "xeon"])
+def test_camrays_follow_the_selected_host(mode):
+    """This host's RSQRTPS (default), or the build host's tables: the GPU's
+    rays equal the oracle's in the same mode and, in the default mode, the
+    reference's own make_camray_pckt running on this host."""
+    L = lib()
+    s = Scene(config=1, detail=0.25, width=256, height=128)
+    L.rend_staticinit(64, 32)
+    n = L.rend_hip_camrays_ref_count(s.rd)
+    d = L.rend_hip_malloc(n * RAY.itemsize)
+    check(L.rend_hip_set_rcp(REND_RCP_XEON_TABLE if mode == "xeon" else REND_RCP_HOST_TABLE))
+    try:
+        say(f"camera rays, {mode} RSQRTPS: {n} rays")
+        check(L.rend_hip_gen_camrays_ref(s.rd, d, 5, None))
+        check(L.rend_hip_sync())
+        got = np.zeros(n, RAY)
+        check(L.rend_hip_memcpy_dtoh(got.ctypes.data, d, n * RAY.itemsize))
+        nblk = (256 // 64) * (128 // 32)
+        want = or_camrays(s, 64, 32, 5, range(nblk), rsqrt=mode).ravel()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        if mode == "host" and reference() is not None:
+            ref = ref_camrays(s, 64, 32, 5, range(nblk)).ravel()
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    finally:
+        L.rend_hip_set_rcp(REND_RCP_HOST_TABLE)
+        L.rend_hip_free(d)
         s.close()

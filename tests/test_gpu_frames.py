@@ -1,5 +1,6 @@
 """The drop-in frame on the GPU against the reference's own frames
-(tests/golden/frames_*.npz, written from the reference's rend_render).
+(tests/golden/frames_*.npz, written from the reference's rend_render on the
+build host, whose RCPSS/RSQRTPS tables the test selects).
 
 rend_render is called from several host threads per frame with main.c's
 frame protocol (blknum, rays, samples; main.c:407-426); rd->buf (the
@@ -17,7 +18,8 @@ import numpy as np
 import pytest
 
 from frames_ref import NAMES, load
-from nodebleed_amd import REND_MODE_NORMALS, Scene, check, lib
+from nodebleed_amd import (REND_MODE_NORMALS, REND_RCP_HOST_TABLE, REND_RCP_XEON_TABLE, Scene,
+                           check, lib)
 
 pytestmark = pytest.mark.gpu
 
@@ -32,6 +34,9 @@ def test_rend_render_matches_the_reference_frames(name):
     L = lib()
     w, h, frames = int(f["width"]), int(f["height"]), int(f["frames"])
     s = Scene(config=int(f["config"]), detail=float(f["detail"]), width=w, height=h)
+    # the fixtures were written on the build host (an Intel Xeon): its
+    # RCPSS/RSQRTPS tables reproduce them on any host
+    check(L.rend_hip_set_rcp(REND_RCP_XEON_TABLE), "rcp")
     try:
         L.rend_staticinit(*(int(b) for b in f["block"]))
         check(L.rend_hip_set_mode(REND_MODE_NORMALS), "mode")
@@ -65,4 +70,5 @@ def test_rend_render_matches_the_reference_frames(name):
             f"(its packet traversal's own hits)")
         assert pk == set(f["packet_diff_px"].tolist())
     finally:
+        L.rend_hip_set_rcp(REND_RCP_HOST_TABLE)
         s.close()

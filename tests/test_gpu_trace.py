@@ -16,7 +16,7 @@ import pytest
 
 from nodebleed_amd import (HIT, RAY, REND_RCP_HOST_TABLE, REND_RCP_IEEE, Scene, check, lib,
                            make_rays)
-from oracle_lib import or_occluded, or_trace
+from oracle_lib import or_occluded, or_trace, reference
 from raygen import aimed_rays, edge_rays, mismatches, random_rays, ulp_diff
 
 pytestmark = pytest.mark.gpu
@@ -123,6 +123,30 @@ def test_wave_scheduling_does_not_change_results(scenes, refill, batch):
     # lane utilisation counters: iterations ran, lanes never above 64 each
     assert lanes[0] > 0 and 0 < lanes[1] <= 64 * lanes[0]
     assert lanes[2] > 0 and lanes[3] > 0
+
+
+@pytest.mark.skipif(reference() is None, reason="reference not built (oracle/_ref)")
+@pytest.mark.parametrize("name", list(SCENES))
+def test_hits_match_the_reference_on_this_host(scenes, name):
+    """The reference's own intersect_tlas (oracle/_ref/libref.so, its
+    sources compiled with -ffp-contract=off) on this host's CPU against the
+    GPU in the default mode, which uses this host's RCPSS captured at
+    set-up: ids, t, u and v bit for bit, whichever x86 vendor runs the
+    reference (Intel and AMD cores differ in RCPSS)."""
+    from oracle_lib import RefScene
+    L = lib()
+    check(L.rend_hip_set_rcp(REND_RCP_HOST_TABLE))
+    s = scenes[name]
+    say(f"reference on this host ({L.rend_hip_rcp_cpu().decode()}): {name}")
+    assert b"rule failed" not in L.rend_hip_rcp_cpu()
+    r = RefScene(s)
+    try:
+        for kind, rays in ray_sets(s).items():
+            want = r.trace(rays)
+            got = s.trace(rays)
+            assert mismatches(got, want) == {"id": 0, "t": 0, "u": 0, "v": 0}, (name, kind)
+    finally:
+        r.close()
 
 
 def test_cube_cracks_match_the_reference_semantics():
