@@ -87,7 +87,7 @@ int          rend_hip_trace(struct rdata *rd, const struct rray *d_rays,
  * Same kernels and results as rend_hip_trace.  The scene must already be on
  * the device (rend_hip_sync_scene after rend_prepstatic/rend_prepdynamic,
  * else REND_ESTATE), n <= 2^31 - 1 rays (else REND_EINVAL), no visit
- * counting.  The traversal stack spills into a static 48 MiB array per
+ * counting.  The traversal stack spills into a static 49 MiB array per
  * device that the library holds until the process ends; every call on a
  * device shares it, so calls that overlap on one device race (calls on
  * different streams are ordered by the library outside graph capture). */

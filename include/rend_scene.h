@@ -73,9 +73,30 @@ int  rs_camera(struct rs_scene *s, const float eye[3], const float target[3],
                float vfov, float focdist, float focangle);
 int  rs_background(struct rs_scene *s, float r, float g, float b);
 
-/* Flatten into rd at w x h with spp samples per pixel per frame. */
+/* Flatten into rd at w x h with spp samples per pixel per frame
+ * (cpy_rdata, upd_rinsts, upd_rcam; main.c:53-195).  A scene with a scene
+ * graph (rs_gltf) is flattened at animation time 0. */
 int  rs_commit(struct rs_scene *s, struct rdata *rd, unsigned int w,
                unsigned int h, unsigned int spp);
+
+/* glTF 2.0 scene (a .gltf file and its one external .bin with the same
+ * name) into the builder, with the reference importer's subset and rules
+ * (gltf.h:4-11, gltf.c, gltfimp.c:330-435): the first scene's node tree
+ * under a root node, meshes of triangle primitives with float POSITION and
+ * NORMAL and u8/u16/u32 indices, materials (baseColorFactor,
+ * metallicFactor, roughnessFactor; emission = emissiveFactor x
+ * KHR_materials_emissive_strength), perspective cameras (camera 0 is
+ * used), node translation/rotation/scale and their keyframe animations
+ * (STEP, LINEAR, CUBICSPLINE).  Every mesh node becomes an instance.
+ * Returns 0, or a negative error with rend_hip_errstr() set. */
+int  rs_gltf(struct rs_scene *s, const char *path);
+
+/* Advance a scene-graph scene to `time` seconds (scene_updanims,
+ * scene_updtransforms, scene_updcams; upd_rinsts and rend_prepdynamic when
+ * a mesh node moved, upd_rcam when a camera did: main.c:253-278).  Returns
+ * a mask of what changed (1 instances and TLAS, 2 camera) or a negative
+ * error. */
+int  rs_animate(struct rs_scene *s, struct rdata *rd, float time);
 
 /* Recompute rd->cam for a camera after rs_commit (rendering w x h). */
 int  rs_set_rcam(struct rdata *rd, const float eye[3], const float target[3],
@@ -104,6 +125,8 @@ struct rs_view {
 	unsigned int   instcnt, tlasofs, width, height, spp, samples, rays;
 	int            blknum;
 	float          bgcol[3];
+	const void    *cam;      /* &rd->cam, camsize bytes */
+	unsigned int   camsize;
 };
 void rs_rdata_view(const struct rdata *rd, struct rs_view *v);
 /* Camera constants of rd->cam: eye, ri, up, fwd (3 each), 1/w, 1/h, aspect,

@@ -47,7 +47,8 @@ class RsView(C.Structure):
                 ("acc", C.c_void_p), ("buf", C.c_void_p), ("instcnt", C.c_uint),
                 ("tlasofs", C.c_uint), ("width", C.c_uint), ("height", C.c_uint),
                 ("spp", C.c_uint), ("samples", C.c_uint), ("rays", C.c_uint),
-                ("blknum", C.c_int), ("bgcol", C.c_float * 3)]
+                ("blknum", C.c_int), ("bgcol", C.c_float * 3), ("cam", C.c_void_p),
+                ("camsize", C.c_uint)]
 
 
 class TraceStats(C.Structure):
@@ -172,6 +173,8 @@ _SIGS = {
     "rs_mtlcount": (C.c_uint, [C.c_void_p]),
     "rs_mesh_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_void_p, C.c_void_p,
                                C.c_uint, C.c_void_p, C.c_uint]),
+    "rs_gltf": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "rs_animate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float]),
     "rs_noise": (C.c_float, [C.c_float, C.c_float, C.c_float, C.c_uint]),
     "rs_instance_glob": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_float)]),
     "rs_rdata_view": (None, [C.c_void_p, C.POINTER(RsView)]),
@@ -247,6 +250,27 @@ class Scene:
         rs = L.rs_new()
         fill(L, rs)
         return cls(None, builder=rs, width=width, height=height, spp=spp)
+
+    @classmethod
+    def from_gltf(cls, path, width=64, height=64, spp=1, bg=(0.4, 0.4, 0.4)):
+        """A glTF 2.0 scene (path.gltf beside path.bin), flattened at time 0."""
+        L = lib()
+        rs = L.rs_new()
+        rc = L.rs_gltf(rs, str(path).encode())
+        if rc != 0:
+            msg = L.rend_hip_errstr().decode()
+            L.rs_free(rs)
+            raise RendError(f"rs_gltf({path}): error {rc}: {msg}")
+        L.rs_background(rs, *bg)
+        return cls(None, builder=rs, width=width, height=height, spp=spp)
+
+    def animate(self, time: float) -> int:
+        """Advances the scene graph to `time`; returns the mask of what
+        changed (1 instances and TLAS, 2 camera)."""
+        rc = self.L.rs_animate(self.rs, self.rd, time)
+        if rc < 0:
+            check(rc, "rs_animate")
+        return rc
 
     def view(self) -> RsView:
         v = RsView()

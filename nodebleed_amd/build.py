@@ -30,7 +30,7 @@ HIPCC = shutil.which("hipcc") or str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
 HOST_SRC = ["host/bvh8.c", "host/rend.c", "host/frame.c", "host/pt.c",
-            "host/scenegen.c", "host/guide.c", "host/approx.c"]
+            "host/scenegen.c", "host/guide.c", "host/approx.c", "host/gltf.c"]
 HIP_SRC = ["kern/trace.hip", "kern/frame.hip", "kern/pt.hip", "kern/guide.hip", "kern/tlas.hip"]
 
 CFLAGS = ["-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-fno-strict-aliasing",

@@ -25,6 +25,39 @@
 #define BVH_HAZ_VDEGEN    4u  /* empty vertical side: reference leaves a count as a node offset */
 #define BVH_HAZ_STACK     8u  /* builder stack would exceed 64 entries (rend.c:639) */
 
+/* ---- scene builder internals (scenegen.c, gltf.c) ----------------------- */
+
+/* Node flags (types.h:4-18) and scene-graph constants (scene.h:63-100). */
+#define RS_MESH       0x0001u
+#define RS_CAM        0x0002u
+#define RS_DYNAMIC    0x0200u
+#define RS_EMISSIVE   0x1000u
+#define RS_REFRACTIVE 0x2000u
+#define RS_TGT_TRANS  0
+#define RS_TGT_ROT    1
+#define RS_TGT_SCALE  2
+#define RS_INTERP_STEP   0
+#define RS_INTERP_LINEAR 1
+#define RS_INTERP_CUBIC  2
+#define RS_DIRTY_TRANSFORM 1
+#define RS_DIRTY_CAM       2
+
+struct rs_scene;
+int      rsi_node(struct rs_scene *s, int parent, int mesh, int cam, unsigned flags,
+                  const float t[3], const float q[4], const float sc[3]);
+unsigned rsi_node_flags(struct rs_scene *s, unsigned node, unsigned set);
+int      rsi_cam(struct rs_scene *s, float vfov, float focdist, float focangle, int node);
+int      rsi_cam_node(struct rs_scene *s, unsigned cam, int node);
+unsigned rsi_mtl_flags(const struct rs_scene *s, unsigned id);
+int      rsi_empty(const struct rs_scene *s);
+int      rsi_mtl_set(struct rs_scene *s, unsigned id, float metallic, float roughness,
+                     unsigned flags, const float *col);
+long     rsi_animdata(struct rs_scene *s, const float *v, unsigned n);
+int      rsi_sampler(struct rs_scene *s, unsigned kcnt, unsigned kofs, unsigned dofs,
+                     int interp);
+int      rsi_track(struct rs_scene *s, unsigned sampler, unsigned node, int tgt);
+int      rsi_instances_from_nodes(struct rs_scene *s);
+
 /* ---- float helpers with the reference's operation order ---------------- */
 
 static inline struct vec3 rv_add(struct vec3 a, struct vec3 b)

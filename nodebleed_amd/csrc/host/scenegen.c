@@ -95,6 +95,33 @@ struct rs_mesh {
 struct rs_inst {
 	unsigned  mesh, flags;
 	float     glob[16];
+	int       node;   /* scene-graph node it follows, or -1 */
+};
+
+/* Scene graph of imported scenes (scene.h:47-130): nodes with transform
+ * components, local and world matrices, keyframe tracks and samplers over
+ * one block of animation data. */
+struct rs_node {
+	int       parent;
+	int       mesh, cam;
+	unsigned  flags;
+	float     trans[3], rot[4], scale[3];
+	float     loc[16], glob[16];
+};
+
+struct rs_track {
+	unsigned  sampler, node, key;
+	int       tgt;    /* RS_TGT_* */
+};
+
+struct rs_sampler {
+	unsigned  kcnt, kofs, dofs;  /* key count, key and value offsets (floats) */
+	int       interp;            /* RS_INTERP_* */
+};
+
+struct rs_cam {
+	float     vfov, focdist, focangle;  /* degrees */
+	int       node;
 };
 
 struct rs_scene {
@@ -109,6 +136,18 @@ struct rs_scene {
 	int             open;
 	float           eye[3], target[3], vfov, focdist, focangle;
 	float           bg[3];
+	/* scene graph (imported scenes) */
+	struct rs_node   *nodes;
+	unsigned          nnode, capnode;
+	struct rs_track  *tracks;
+	unsigned          ntrack, captrack;
+	struct rs_sampler *samplers;
+	unsigned          nsampler, capsampler;
+	float            *animdata;
+	unsigned          nanim, capanim;   /* floats */
+	struct rs_cam    *cams;
+	unsigned          ncam, capcam;
+	unsigned          dirty;            /* RS_DIRTY_* */
 };
 
 #define GROW(p, n, cap, extra) do { \
@@ -150,6 +189,11 @@ void rs_free(struct rs_scene *s)
 	free(s->meshes);
 	free(s->mtls);
 	free(s->insts);
+	free(s->nodes);
+	free(s->tracks);
+	free(s->samplers);
+	free(s->animdata);
+	free(s->cams);
 	free(s);
 }
 
@@ -597,7 +641,258 @@ int rs_instance(struct rs_scene *s, unsigned int mesh, const float t[3],
 	m4_mul(in->glob, mt, in->glob);
 	in->mesh = mesh;
 	in->flags = flags | 0x0001u;  /* MESH */
+	in->node = -1;
 	return (int)s->ninst++;
+}
+
+/* ---- scene graph and keyframe animation (scene.c:176-399) ----------------- */
+
+/* Local matrix T * R(q) * S of transform components (combine_tfcomp,
+ * scene.c:176-185). */
+static void combine_trs(float d[16], const float t[3], const float q[4], const float sc[3])
+{
+	float ms[16] = {0}, mr[16], mt[16] = {0};
+	ms[0] = sc[0];
+	ms[5] = sc[1];
+	ms[10] = sc[2];
+	ms[15] = 1.0f;
+	m4_fromquat(mr, q[0], q[1], q[2], q[3]);
+	mt[0] = mt[5] = mt[10] = mt[15] = 1.0f;
+	mt[3] = t[0];
+	mt[7] = t[1];
+	mt[11] = t[2];
+	m4_mul(d, mr, ms);
+	m4_mul(d, mt, d);
+}
+
+int rsi_node(struct rs_scene *s, int parent, int mesh, int cam, unsigned flags,
+             const float t[3], const float q[4], const float sc[3])
+{
+	if (parent >= (int)s->nnode)
+		return -1;
+	GROW(s->nodes, s->nnode, s->capnode, 1);
+	struct rs_node *n = &s->nodes[s->nnode];
+	memset(n, 0, sizeof(*n));
+	n->parent = parent;
+	n->mesh = mesh;
+	n->cam = cam;
+	n->flags = flags;
+	memcpy(n->trans, t, sizeof(n->trans));
+	memcpy(n->rot, q, sizeof(n->rot));
+	memcpy(n->scale, sc, sizeof(n->scale));
+	combine_trs(n->loc, t, q, sc);
+	memcpy(n->glob, n->loc, sizeof(n->glob));
+	return (int)s->nnode++;
+}
+
+unsigned rsi_node_flags(struct rs_scene *s, unsigned node, unsigned set)
+{
+	if (node >= s->nnode)
+		return 0;
+	s->nodes[node].flags |= set;
+	return s->nodes[node].flags;
+}
+
+int rsi_cam(struct rs_scene *s, float vfov, float focdist, float focangle, int node)
+{
+	GROW(s->cams, s->ncam, s->capcam, 1);
+	struct rs_cam *c = &s->cams[s->ncam];
+	c->vfov = vfov;
+	c->focdist = focdist;
+	c->focangle = focangle;
+	c->node = node;
+	return (int)s->ncam++;
+}
+
+int rsi_cam_node(struct rs_scene *s, unsigned cam, int node)
+{
+	if (cam >= s->ncam)
+		return -1;
+	s->cams[cam].node = node;
+	return 0;
+}
+
+int rsi_mtl_set(struct rs_scene *s, unsigned id, float metallic, float roughness,
+                unsigned flags, const float *col)
+{
+	if (id >= s->nmtl)
+		return -1;
+	struct rmtl *m = &s->mtls[id];
+	m->metallic = metallic;
+	m->roughness = roughness;
+	m->flags = flags;
+	if (col) {
+		m->col.x = col[0];
+		m->col.y = col[1];
+		m->col.z = col[2];
+	}
+	return 0;
+}
+
+unsigned rsi_mtl_flags(const struct rs_scene *s, unsigned id)
+{
+	return id < s->nmtl ? s->mtls[id].flags : 0u;
+}
+
+/* Nothing added yet: no materials, meshes, instances, nodes or cameras. */
+int rsi_empty(const struct rs_scene *s)
+{
+	return !s->nmtl && !s->nmesh && !s->ninst && !s->nnode && !s->ncam && !s->open;
+}
+
+/* Appends floats to the animation data; returns their offset or -1. */
+long rsi_animdata(struct rs_scene *s, const float *v, unsigned n)
+{
+	GROW(s->animdata, s->nanim, s->capanim, n ? n : 1);
+	memcpy(&s->animdata[s->nanim], v, (size_t)n * sizeof(float));
+	long at = (long)s->nanim;
+	s->nanim += n;
+	return at;
+}
+
+int rsi_sampler(struct rs_scene *s, unsigned kcnt, unsigned kofs, unsigned dofs, int interp)
+{
+	GROW(s->samplers, s->nsampler, s->capsampler, 1);
+	s->samplers[s->nsampler] = (struct rs_sampler){kcnt, kofs, dofs, interp};
+	return (int)s->nsampler++;
+}
+
+/* A track marks its node as animated (scene_inittrack, scene.c:233-246). */
+int rsi_track(struct rs_scene *s, unsigned sampler, unsigned node, int tgt)
+{
+	if (node >= s->nnode)
+		return -1;
+	GROW(s->tracks, s->ntrack, s->captrack, 1);
+	s->tracks[s->ntrack] = (struct rs_track){sampler, node, 0, tgt};
+	s->nodes[node].flags |= RS_DYNAMIC;
+	return (int)s->ntrack++;
+}
+
+/* Instance of every mesh node, in node order (cpy_rdata, main.c:98-108). */
+int rsi_instances_from_nodes(struct rs_scene *s)
+{
+	for (unsigned i = 0; i < s->nnode; i++) {
+		struct rs_node *n = &s->nodes[i];
+		if (!(n->flags & RS_MESH))
+			continue;
+		if (n->mesh < 0 || (unsigned)n->mesh >= s->nmesh || s->ninst >= 4096)
+			return -1;
+		GROW(s->insts, s->ninst, s->capinst, 1);
+		struct rs_inst *in = &s->insts[s->ninst++];
+		in->mesh = (unsigned)n->mesh;
+		in->flags = n->flags;
+		in->node = (int)i;
+		memcpy(in->glob, n->glob, sizeof(in->glob));
+	}
+	return 0;
+}
+
+/* Key interval for time t, searched from the track's last key
+ * (find_key, scene.c:248-254: a later time never moves it back). */
+static unsigned find_key(const float *keys, unsigned last, unsigned cnt, float t)
+{
+	for (int i = (int)last; i < (int)cnt - 1; i++)
+		if (t <= keys[i])
+			return (unsigned)(i - 1 > 0 ? i - 1 : 0);
+	return cnt - 2;
+}
+
+static void interp_linear(float *d, const float *a, const float *b, float t)
+{
+	for (int k = 0; k < 3; k++)
+		d[k] = (1.0f - t) * a[k] + t * b[k];
+}
+
+/* Spherical interpolation of unit quaternions along the shorter arc
+ * (spher_lerp, scene.c:263-278). */
+static void interp_slerp(float *d, const float *a, const float *b, float t)
+{
+	float c = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+	float ang = acosf(c);
+	float isin = 1.0f / sinf(ang);
+	float e = sinf(ang * (1.0f - t)) * isin;
+	float f = (c / fabsf(c)) * sinf(ang * t) * isin;
+	for (int k = 0; k < 4; k++)
+		d[k] = e * a[k] + f * b[k];
+}
+
+/* Cubic Hermite spline over (in-tangent, value, out-tangent) keyframes,
+ * tangents scaled by the interval (glTF CUBICSPLINE; cubic,
+ * scene.c:280-300). */
+static void interp_cubic(float *d, const float *a, const float *b, float t, float dur,
+                         unsigned n)
+{
+	float t2 = t * t, t3 = t2 * t;
+	float c0 = 2 * t3 - 3 * t2 + 1;
+	float c1 = dur * (t3 - 2 * t2 + t);
+	float c2 = -2.0f * t3 + 3 * t2;
+	float c3 = dur * (t3 - t2);
+	for (unsigned k = 0; k < n; k++)
+		d[k] = c0 * a[n + k] + c1 * a[2 * n + k] + c2 * b[n + k] + c3 * b[k];
+}
+
+/* Tracks at time t into the nodes' transform components and local
+ * matrices (scene_updanims, scene.c:302-384); sets RS_DIRTY_TRANSFORM for
+ * mesh nodes, RS_DIRTY_CAM for camera nodes. */
+static void upd_anims(struct rs_scene *s, float time)
+{
+	for (unsigned j = 0; j < s->ntrack; j++) {
+		struct rs_track *tr = &s->tracks[j];
+		const struct rs_sampler *sa = &s->samplers[tr->sampler];
+		const float *keys = &s->animdata[sa->kofs];
+		unsigned k = find_key(keys, tr->key, sa->kcnt, time);
+		tr->key = k;
+		float t0 = keys[k], t1 = keys[k + 1];
+		float tc = rf_max(rf_min(time, t1), t0);
+		float dur = t1 - t0;
+		float t = (tc - t0) / dur;
+		unsigned comp = tr->tgt == RS_TGT_ROT ? 4 : 3;
+		unsigned elem = sa->interp == RS_INTERP_CUBIC ? 3 : 1;
+		const float *v0 = &s->animdata[sa->dofs + elem * comp * k];
+		const float *v1 = &s->animdata[sa->dofs + elem * comp * (k + 1)];
+		float v[4];
+		if (sa->interp == RS_INTERP_STEP) {
+			for (unsigned i = 0; i < comp; i++)
+				v[i] = v0[i];
+		} else if (sa->interp == RS_INTERP_LINEAR) {
+			if (comp == 4)
+				interp_slerp(v, v0, v1, t);
+			else
+				interp_linear(v, v0, v1, t);
+		} else {
+			interp_cubic(v, v0, v1, t, dur, comp);
+			if (comp == 4) {
+				float l = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] +
+				  v[3] * v[3]);
+				for (int i = 0; i < 4; i++)
+					v[i] *= l;
+			}
+		}
+		struct rs_node *n = &s->nodes[tr->node];
+		float *dst = tr->tgt == RS_TGT_TRANS ? n->trans :
+		  (tr->tgt == RS_TGT_ROT ? n->rot : n->scale);
+		for (unsigned i = 0; i < comp; i++)
+			dst[i] = v[i];
+		/* the local matrix once per run of tracks on one node */
+		if (j + 1 < s->ntrack && s->tracks[j + 1].node == tr->node)
+			continue;
+		combine_trs(n->loc, n->trans, n->rot, n->scale);
+		if (n->flags & RS_MESH)
+			s->dirty |= RS_DIRTY_TRANSFORM;
+		else if (n->flags & RS_CAM)
+			s->dirty |= RS_DIRTY_CAM;
+	}
+}
+
+/* World matrices, parents before children (scene_updtransforms,
+ * scene.c:386-399: node 0 is the root). */
+static void upd_transforms(struct rs_scene *s)
+{
+	if (!s->nnode)
+		return;
+	memcpy(s->nodes[0].glob, s->nodes[0].loc, sizeof(s->nodes[0].glob));
+	for (unsigned i = 1; i < s->nnode; i++)
+		m4_mul(s->nodes[i].glob, s->nodes[s->nodes[i].parent].glob, s->nodes[i].loc);
 }
 
 int rs_camera(struct rs_scene *s, const float eye[3], const float target[3],
@@ -645,12 +940,11 @@ static struct vec3 v3cross(struct vec3 a, struct vec3 b)
 	  a.x * b.y - a.y * b.x);
 }
 
-int rs_set_rcam(struct rdata *rd, const float eye[3], const float target[3],
-                float vfov, float focdist, float focangle, unsigned int w,
-                unsigned int h)
+/* rcam of a camera at e looking along the unit vector fwd (calc_cam,
+ * scene.c:401-409, then upd_rcam, main.c:159-195). */
+static void set_rcam(struct rdata *rd, struct vec3 e, struct vec3 fwd, float vfov,
+                     float focdist, float focangle, unsigned int w, unsigned int h)
 {
-	struct vec3 e = v3(eye[0], eye[1], eye[2]);
-	struct vec3 fwd = v3unit(rv_sub(v3(target[0], target[1], target[2]), e));
 	struct vec3 ri = v3unit(v3cross(fwd, v3(0.0f, 1.0f, 0.0f)));
 	struct vec3 up = v3unit(v3cross(ri, fwd));
 	float fw = (float)w, fh = (float)h;
@@ -686,7 +980,57 @@ int rs_set_rcam(struct rdata *rd, const float eye[3], const float target[3],
 	c->fovfdist8 = _mm256_set1_ps(2.0f * tanvfov * focdist);
 	c->focangle8 = _mm256_set1_ps(tanfangle);
 	c->focrad8 = _mm256_set1_ps(tanfangle * focdist);
+}
+
+int rs_set_rcam(struct rdata *rd, const float eye[3], const float target[3],
+                float vfov, float focdist, float focangle, unsigned int w,
+                unsigned int h)
+{
+	struct vec3 e = v3(eye[0], eye[1], eye[2]);
+	struct vec3 fwd = v3unit(rv_sub(v3(target[0], target[1], target[2]), e));
+	set_rcam(rd, e, fwd, vfov, focdist, focangle, w, h);
 	return 0;
+}
+
+/* The scene graph's current camera (camera 0) from its node's world matrix:
+ * eye = translation, forward = -z column (calc_cam, scene.c:401-409). */
+static void node_rcam(struct rs_scene *s, struct rdata *rd)
+{
+	const struct rs_cam *c = &s->cams[0];
+	const float *m = s->nodes[c->node >= 0 && (unsigned)c->node < s->nnode ?
+	  (unsigned)c->node : 0].glob;
+	struct vec3 e = v3(m[3], m[7], m[11]);
+	struct vec3 col2 = v3(m[2], m[6], m[10]);
+	struct vec3 fwd = v3unit(v3(-col2.x, -col2.y, -col2.z));
+	set_rcam(rd, e, fwd, c->vfov, c->focdist, c->focangle, rd->width, rd->height);
+}
+
+static int inst_transforms(struct rs_scene *s, struct rdata *rd);
+
+int rs_animate(struct rs_scene *s, struct rdata *rd, float time)
+{
+	if (!s->nnode)
+		return rl_fail(REND_EINVAL, "rs_animate: the scene has no scene graph");
+	int changed = 0;
+	upd_anims(s, time);
+	if (s->dirty & RS_DIRTY_TRANSFORM) {
+		upd_transforms(s);
+		int r = inst_transforms(s, rd);
+		if (r)
+			return r;
+		rend_prepdynamic(rd);
+		changed |= RS_DIRTY_TRANSFORM;
+	}
+	if (s->dirty & RS_DIRTY_CAM) {
+		/* as in the reference's update (main.c:267-272), world matrices
+		 * are recomputed only when a mesh node moved: an animated camera
+		 * alone keeps its previous world matrix */
+		if (s->ncam)
+			node_rcam(s, rd);
+		changed |= RS_DIRTY_CAM;
+	}
+	s->dirty = 0;
+	return changed;
 }
 
 /* World AABB of an instance from its BLAS root (upd_rinsts, main.c:117-157). */
@@ -773,16 +1117,41 @@ int rs_commit(struct rs_scene *s, struct rdata *rd, unsigned int w,
 	free(triofs);
 
 	rend_prepstatic(rd);
-	for (unsigned i = 0; i < s->ninst; i++) {
-		float inv[16];
-		if (m4_inv(inv, s->insts[i].glob))
-			return rl_fail(REND_EINVAL, "instance %u: singular transform", i);
-		memcpy(rd->insts[i].globinv, inv, sizeof(rd->insts[i].globinv));
-		inst_aabb(rd, i, s->insts[i].glob);
+	if (s->nnode) {
+		/* the imported scene at time 0 (main.c:243-245) */
+		upd_anims(s, 0.0f);
+		upd_transforms(s);
 	}
+	int r = inst_transforms(s, rd);
+	if (r)
+		return r;
 	rend_prepdynamic(rd);
 	rend_resaccum(rd, w, h);
-	rs_set_rcam(rd, s->eye, s->target, s->vfov, s->focdist, s->focangle, w, h);
+	if (s->nnode && s->ncam)
+		node_rcam(s, rd);
+	else
+		rs_set_rcam(rd, s->eye, s->target, s->vfov, s->focdist, s->focangle, w, h);
+	/* the first rs_animate refreshes instances and camera
+	 * (scene_init's dirty flags, scene.c:78) */
+	s->dirty = RS_DIRTY_TRANSFORM | RS_DIRTY_CAM;
+	return 0;
+}
+
+/* Instance inverse matrices and world boxes from their transforms
+ * (upd_rinsts, main.c:117-157); instances of scene-graph nodes take the
+ * node's world matrix. */
+static int inst_transforms(struct rs_scene *s, struct rdata *rd)
+{
+	for (unsigned i = 0; i < s->ninst; i++) {
+		struct rs_inst *in = &s->insts[i];
+		if (in->node >= 0)
+			memcpy(in->glob, s->nodes[in->node].glob, sizeof(in->glob));
+		float inv[16];
+		if (m4_inv(inv, in->glob))
+			return rl_fail(REND_EINVAL, "instance %u: singular transform", i);
+		memcpy(rd->insts[i].globinv, inv, sizeof(rd->insts[i].globinv));
+		inst_aabb(rd, i, in->glob);
+	}
 	return 0;
 }
 
@@ -1037,6 +1406,8 @@ void rs_rdata_view(const struct rdata *rd, struct rs_view *v)
 	v->bgcol[0] = rd->bgcol.x;
 	v->bgcol[1] = rd->bgcol.y;
 	v->bgcol[2] = rd->bgcol.z;
+	v->cam = &rd->cam;
+	v->camsize = (unsigned)sizeof(rd->cam);
 }
 
 void rs_rdata_camera(const struct rdata *rd, float out[18])

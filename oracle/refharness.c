@@ -11,16 +11,28 @@
  *   ref_occluded: intersect_any_tlas per ray (rend.c:2883)
  *   ref_render / ref_render_single: whole frames of rend_render
  *              (rend.c:3603-3708) and of its single-ray shading path
- * The instance AABB step restates upd_rinsts (main.c:117-157) because main.c
- * needs SDL; it calls the reference's own mulpos.
+ *   ref_gltf_*: a glTF scene through the reference's importer and scene
+ *              graph (gltfimp.c imp_gltf_f, scene.c scene_updanims /
+ *              scene_updtransforms / scene_updcams) into an rdata, set up
+ *              and advanced as main.c's init() and update() do
+ *              (main.c:208-278)
+ * The instance AABB step restates upd_rinsts (main.c:117-157), and the glTF
+ * path restates cpy_rdata and upd_rcam (main.c:53-110, 159-195), because
+ * main.c needs SDL; they call the reference's own mulpos, rend_* and scene_*
+ * functions.
  */
 #include <float.h>
+#include <math.h>
 #include <pthread.h>
 #include <stdbool.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include "gltfimp.h"
+#include "mat4.h"
 #include "rend.h"
+#include "scene.h"
+#include "types.h"
 #include "util.h"
 #include "vec3.h"
 
@@ -317,4 +329,208 @@ unsigned ref_render_single(struct rdata *rd, unsigned bszx, unsigned bszy, unsig
 	copy_out(rd, buf, acc);
 	rend_staticrelease();
 	return rays;
+}
+
+/* ---- glTF scenes through the reference's scene graph ----------------------- */
+
+struct ref_gltf {
+	struct scene  s;
+	struct rdata *rd;
+	unsigned      ntris;
+};
+
+/* World AABB of instance `inst` from its BLAS root and world matrix m
+ * (upd_rinsts, main.c:117-157). */
+static void inst_box(struct rdata *rd, unsigned inst, float *m)
+{
+	struct rinst *ri = &rd->insts[inst];
+	struct bnode8 *n = &rd->bnodes[ri->triofs << 1];
+	float lo[3] = {bcmin8(n->minx8)[0], bcmin8(n->miny8)[0], bcmin8(n->minz8)[0]};
+	float hi[3] = {bcmax8(n->maxx8)[0], bcmax8(n->maxy8)[0], bcmax8(n->maxz8)[0]};
+	struct vec3 mi = {FLT_MAX, FLT_MAX, FLT_MAX};
+	struct vec3 ma = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	for (unsigned char c = 0; c < 8; c++) {
+		struct vec3 v;
+		mulpos(&v.x, &v.y, &v.z, m, c & 1 ? lo[0] : hi[0], c & 2 ? lo[1] : hi[1],
+		  c & 4 ? lo[2] : hi[2]);
+		mi = vec3_min(mi, v);
+		ma = vec3_max(ma, v);
+	}
+	rd->aabbs[inst] = (struct aabb){.min = mi, .max = ma};
+}
+
+static void gltf_insts(struct ref_gltf *g)
+{
+	struct scene *s = &g->s;
+	for (unsigned j = 0; j < s->nodecnt; j++) {
+		struct obj *o = &s->objs[j];
+		if (!hasflags(o->flags, MESH))
+			continue;
+		float inv[16];
+		mat4_inv(inv, s->tfmats[j].glob);
+		memcpy(g->rd->insts[o->instid].globinv, inv, sizeof(g->rd->insts[0].globinv));
+		inst_box(g->rd, (unsigned)o->instid, s->tfmats[j].glob);
+	}
+}
+
+/* rcam of the scene's current camera (upd_rcam, main.c:159-195). */
+static void gltf_rcam(struct ref_gltf *g)
+{
+	struct cam *c = &g->s.cams[g->s.currcam];
+	struct rcam *rc = &g->rd->cam;
+	float w = g->rd->width, h = g->rd->height;
+	float aspect = w / h;
+	float tv = tanf(0.5f * c->vfov * PI / 180.0f);
+	float tf = tanf(0.5f * c->focangle * PI / 180.0f);
+	memset(rc, 0, sizeof(*rc));
+	rc->eye = c->eye;
+	rc->tanvfov = tv;
+	rc->ri = c->ri;
+	rc->focdist = c->focdist;
+	rc->up = c->up;
+	rc->tanfangle = tf;
+	rc->fwd = c->fwd;
+	rc->aspect = aspect;
+	rc->eyex8 = _mm256_set1_ps(c->eye.x);
+	rc->eyey8 = _mm256_set1_ps(c->eye.y);
+	rc->eyez8 = _mm256_set1_ps(c->eye.z);
+	rc->rix8 = _mm256_set1_ps(c->ri.x);
+	rc->riy8 = _mm256_set1_ps(c->ri.y);
+	rc->riz8 = _mm256_set1_ps(c->ri.z);
+	rc->upx8 = _mm256_set1_ps(c->up.x);
+	rc->upy8 = _mm256_set1_ps(c->up.y);
+	rc->upz8 = _mm256_set1_ps(c->up.z);
+	rc->fwdx8 = _mm256_set1_ps(c->fwd.x);
+	rc->fwdy8 = _mm256_set1_ps(c->fwd.y);
+	rc->fwdz8 = _mm256_set1_ps(c->fwd.z);
+	rc->rw8 = _mm256_set1_ps(1.0f / w);
+	rc->rh8 = _mm256_set1_ps(1.0f / h);
+	rc->aspect8 = _mm256_set1_ps(aspect);
+	rc->fdist8 = _mm256_set1_ps(c->focdist);
+	rc->fovfdist8 = _mm256_set1_ps(2.0f * tv * c->focdist);
+	rc->focangle8 = _mm256_set1_ps(tf);
+	rc->focrad8 = _mm256_set1_ps(tf * c->focdist);
+}
+
+/* Import and set up as main.c's init() (main.c:208-251); bg is the
+ * background colour the application sets. */
+struct ref_gltf *ref_gltf_open(const char *path, unsigned w, unsigned h, unsigned spp,
+                               float bgr, float bgg, float bgb)
+{
+	if (!g_init) {
+		rend_staticinit(64, 32);
+		g_init = 1;
+	}
+	struct ref_gltf *g = calloc(1, sizeof(*g));
+	struct scene *s = &g->s;
+	imp_gltf_f(s, (char *)path);
+	s->bgcol = (struct vec3){bgr, bgg, bgb};
+	unsigned ninst = 0;
+	for (unsigned i = 0; i < s->meshcnt; i++)
+		g->ntris += s->meshes[i].icnt / 3;
+	for (unsigned i = 0; i < s->nodecnt; i++)
+		ninst += hasflags(s->objs[i].flags, MESH);
+	g->rd = aligned_alloc(64, (sizeof(*g->rd) + 63) & ~(size_t)63);
+	memset(g->rd, 0, sizeof(*g->rd));
+	struct rdata *rd = g->rd;
+	rend_init(rd, s->mtlmax, g->ntris, ninst, spp);
+	rend_resaccum(rd, w, h);
+	rd->buf = calloc((size_t)w * h, sizeof(unsigned));
+
+	/* cpy_rdata: triangles mesh by mesh and material run by run */
+	unsigned *triofs = calloc(s->meshcnt ? s->meshcnt : 1, sizeof(unsigned));
+	struct rtri *rt = rd->tris;
+	struct rnrm *rn = rd->nrms;
+	unsigned ofs = 0;
+	for (unsigned k = 0; k < s->meshcnt; k++) {
+		struct mesh *m = &s->meshes[k];
+		const unsigned *ip = m->inds;
+		for (unsigned j = 0; j < m->mcnt; j++)
+			for (unsigned i = 0; i < m->mtls[j].tricnt; i++, ip += 3, rt++, rn++) {
+				memcpy(&rt->v0, &m->vrts[ip[0]], sizeof(rt->v0));
+				memcpy(&rt->v1, &m->vrts[ip[1]], sizeof(rt->v1));
+				memcpy(&rt->v2, &m->vrts[ip[2]], sizeof(rt->v2));
+				memcpy(&rn->n0, &m->nrms[ip[0]], sizeof(rn->n0));
+				memcpy(&rn->n1, &m->nrms[ip[1]], sizeof(rn->n1));
+				memcpy(&rn->n2, &m->nrms[ip[2]], sizeof(rn->n2));
+				rn->mtlid = m->mtls[j].mtlid;
+			}
+		triofs[k] = ofs;
+		ofs += m->icnt / 3;
+	}
+	/* struct mtl (28 bytes) into struct rmtl (32): the pad is left out */
+	for (unsigned i = 0; i < s->mtlcnt; i++) {
+		memset(&rd->mtls[i], 0, sizeof(rd->mtls[i]));
+		memcpy(&rd->mtls[i], &s->mtls[i], sizeof(s->mtls[i]));
+	}
+	unsigned cnt = 0;
+	for (unsigned i = 0; i < s->nodecnt; i++) {
+		struct obj *o = &s->objs[i];
+		if (!hasflags(o->flags, MESH))
+			continue;
+		rd->insts[cnt] = (struct rinst){.flags = o->flags, .triofs = triofs[o->objid],
+		  .tricnt = s->meshes[o->objid].icnt / 3};
+		o->instid = (int)cnt++;
+	}
+	rd->instcnt = cnt;
+	rd->bgcol = s->bgcol;
+	free(triofs);
+
+	rend_prepstatic(rd);
+	scene_updanims(s, 0.0f);
+	scene_updtransforms(s);
+	scene_updcams(s);
+	return g;
+}
+
+/* One update() at `time` (main.c:253-278); returns 1 when instances and the
+ * TLAS were refreshed, | 2 when the camera was. */
+int ref_gltf_update(struct ref_gltf *g, float time)
+{
+	struct scene *s = &g->s;
+	int r = 0;
+	scene_updanims(s, time);
+	if (hasflags(s->dirty, TRANSFORM)) {
+		scene_updtransforms(s);
+		gltf_insts(g);
+		rend_prepdynamic(g->rd);
+		r |= 1;
+	}
+	if (hasflags(s->dirty, CAM)) {
+		scene_updcams(s);
+		gltf_rcam(g);
+		r |= 2;
+	}
+	clrflags(&s->dirty, TRANSFORM | CAM);
+	return r;
+}
+
+struct rdata *ref_gltf_rdata(struct ref_gltf *g) { return g->rd; }
+
+/* A new accumulation (main.c:275-278 clears it after every change). */
+void ref_gltf_restart(struct ref_gltf *g)
+{
+	g->rd->samples = 0;
+	g->rd->blknum = 0;
+	g->rd->rays = 0;
+}
+
+/* mtls, tris, nodes, tracks, samplers of the imported scene. */
+void ref_gltf_counts(struct ref_gltf *g, unsigned out[5])
+{
+	out[0] = g->s.mtlcnt;
+	out[1] = g->ntris;
+	out[2] = g->s.nodecnt;
+	out[3] = g->s.trackcnt;
+	out[4] = g->s.samplercnt;
+}
+
+void ref_gltf_close(struct ref_gltf *g)
+{
+	free(g->rd->buf);
+	g->rd->buf = NULL;
+	rend_release(g->rd);
+	free(g->rd);
+	scene_release(&g->s);
+	free(g);
 }

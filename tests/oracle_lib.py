@@ -178,8 +178,104 @@ def reference(fast: bool = False) -> C.CDLL | None:
     L.ref_render_single.restype = C.c_uint
     L.ref_render_single.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_uint, C.c_void_p,
                                     C.c_void_p]
+    L.ref_gltf_open.restype = C.c_void_p
+    L.ref_gltf_open.argtypes = [C.c_char_p, C.c_uint, C.c_uint, C.c_uint, C.c_float,
+                                C.c_float, C.c_float]
+    L.ref_gltf_update.restype = C.c_int
+    L.ref_gltf_update.argtypes = [C.c_void_p, C.c_float]
+    L.ref_gltf_rdata.restype = C.c_void_p
+    L.ref_gltf_rdata.argtypes = [C.c_void_p]
+    L.ref_gltf_counts.restype = None
+    L.ref_gltf_counts.argtypes = [C.c_void_p, C.POINTER(C.c_uint)]
+    L.ref_gltf_restart.restype = None
+    L.ref_gltf_restart.argtypes = [C.c_void_p]
+    L.ref_gltf_close.restype = None
+    L.ref_gltf_close.argtypes = [C.c_void_p]
     _ref[path] = L
     return L
+
+
+class RefGltf:
+    """A glTF file through the reference's importer and scene-graph update
+    (main.c init() and update(), restated in oracle/refharness.c)."""
+
+    def __init__(self, path, width, height, spp, bg):
+        self.L = reference()
+        self.g = self.L.ref_gltf_open(str(path).encode(), width, height, spp, *bg)
+        self.rd = self.L.ref_gltf_rdata(self.g)
+        cnt = (C.c_uint * 5)()
+        self.L.ref_gltf_counts(self.g, cnt)
+        self.nmtls, self.ntris, self.nnodes, self.ntracks, self.nsamplers = list(cnt)
+
+    def update(self, time: float) -> int:
+        return self.L.ref_gltf_update(self.g, time)
+
+    def frames(self, width, height, frames, block=(64, 32), threads=4, single=False):
+        """A new accumulation of `frames` frames of the reference's
+        rend_render (single=False) or of its block walk traced with its
+        single-ray trace_rays (single=True): (ARGB h x w, acc h x w x 3,
+        rays)."""
+        self.L.ref_gltf_restart(self.g)
+        buf = np.zeros(width * height, np.uint32)
+        acc = np.zeros(width * height * 3, np.float32)
+        rd = C.c_void_p(self.rd)
+        if single:
+            n = self.L.ref_render_single(rd, block[0], block[1], frames, ptr(buf), ptr(acc))
+        else:
+            n = self.L.ref_render(rd, block[0], block[1], frames, threads, ptr(buf), ptr(acc))
+        return buf.reshape(height, width), acc.reshape(height, width, 3), n
+
+    def close(self):
+        if self.g:
+            self.L.ref_gltf_close(self.g)
+            self.g = None
+
+
+_RNRM = np.dtype([("n0", "<f4", 3), ("mtlid", "<u4"), ("n1", "<f4", 3), ("p0", "<u4"),
+                  ("n2", "<f4", 3), ("p1", "<u4")])
+
+
+def rdata_parts(rd: int, ntris: int, nmtls: int) -> list[tuple[str, bytes]]:
+    """Every part of an rdata (ours or the reference's: one layout) that
+    the scene set-up and its updates write, as (label, bytes): triangles
+    and normals without their pad words, materials without theirs,
+    instances, instance boxes, camera, background and counts, each BLAS
+    reservation and the TLAS."""
+    from nodebleed_amd import AABB, RINST, RTRI, RsView, lib
+    L = lib()
+    v = RsView()
+    L.rs_rdata_view(C.c_void_p(rd), C.byref(v))
+
+    def arr(p, dtype, n):
+        if not p or n == 0:
+            return np.zeros(0, dtype)
+        return np.frombuffer((C.c_char * (n * np.dtype(dtype).itemsize)).from_address(p),
+                             dtype=dtype, count=n).copy()
+
+    tris = arr(v.tris, RTRI, ntris)
+    nrms = arr(v.nrms, _RNRM, ntris)
+    mtls = arr(v.mtls, np.uint8, nmtls * 32).reshape(nmtls, 32)[:, :28]
+    insts = arr(v.insts, RINST, v.instcnt)
+    boxes = arr(v.aabbs, AABB, v.instcnt)
+    cam = arr(v.cam, np.uint8, v.camsize)
+    meta = np.array([v.instcnt, v.tlasofs, v.width, v.height, v.spp], np.uint32)
+    parts = [("tris", b"".join(tris[f].tobytes() for f in ("v0", "v1", "v2"))),
+             ("nrms", b"".join(nrms[f].tobytes() for f in ("n0", "mtlid", "n1", "n2"))),
+             ("mtls", mtls.tobytes()), ("insts", insts.tobytes()),
+             ("aabbs", boxes["min"].tobytes() + boxes["max"].tobytes()),
+             ("cam", cam.tobytes()),
+             ("meta", meta.tobytes() + np.array(v.bgcol, np.float32).tobytes())]
+    for tofs in sorted(set(insts["triofs"].tolist())):
+        tc = int(insts["tricnt"][insts["triofs"] == tofs][0])
+        parts.append((f"blas@{tofs}", arr(v.bnodes + tofs * 512, np.uint8, tc * 512).tobytes()))
+    parts.append(("tlas", arr(v.bnodes + v.tlasofs * 256, np.uint8,
+                              2 * v.instcnt * 256).tobytes()))
+    return parts
+
+
+def rdata_digests(rd: int, ntris: int, nmtls: int) -> list[tuple[str, str]]:
+    import hashlib
+    return [(k, hashlib.sha256(b).hexdigest()) for k, b in rdata_parts(rd, ntris, nmtls)]
 
 
 class RefScene:
