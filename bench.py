@@ -406,9 +406,12 @@ def guide_equal_error(a):
         return el, download(k * spf), curve
 
     if a.guide_profile_frames:
-        # guided frames only (a profile of their kernels), no error protocol
-        el, _, _ = run(a.guide_profile_frames, True, tag="guided")
-        log(f"guided: {a.guide_profile_frames} frames in {el:.2f} s "
+        # guided (or, with --guide-profile-plain, plain) frames only, for a
+        # profile of their kernels; no error protocol
+        guided = not a.guide_profile_plain
+        tag = "guided" if guided else "plain"
+        el, _, _ = run(a.guide_profile_frames, guided, tag=tag)
+        log(f"{tag}: {a.guide_profile_frames} frames in {el:.2f} s "
             f"({el * 1e3 / a.guide_profile_frames:.1f} ms per frame)")
         L.rend_hip_guide_disable(scene.rd)
         scene.close()
@@ -495,6 +498,8 @@ def main():
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     ap.add_argument("--guide-profile-frames", type=int, default=0,
                     help="--mode guide: only render this many guided frames (for profiling)")
+    ap.add_argument("--guide-profile-plain", action="store_true",
+                    help="with --guide-profile-frames: plain frames instead of guided ones")
     ap.add_argument("--no-guide", action="store_true",
                     help="trace mode: leave out the guided/plain equal-error run")
     ap.add_argument("--no-graphs", action="store_true",

@@ -228,7 +228,8 @@ int          rend_hip_guide_warmup(struct rdata *rd, unsigned int frames, unsign
 int          rend_hip_guide_info(struct rdata *rd, float out[4]);
 /* Test access on host buffers: all parameters (GD_NPARAM floats, get/set),
  * raw network outputs for positions+normals (6 floats each, 64 outputs per
- * point), and the loss gradient over training records. */
+ * point), and the loss gradient over training records (its hash-grid part
+ * accumulated in bf16 pairs on the device, returned as fp32). */
 int          rend_hip_guide_params(struct rdata *rd, float *params, int set);
 int          rend_hip_guide_forward_host(struct rdata *rd, const float *pn,
                                          unsigned int n, float *out);

@@ -34,7 +34,8 @@ int rk_guide_train(const struct PtArgs *a, const void *grid16, const void *w16,
                    const float *master, const void *rec, const unsigned *counts,
                    unsigned cap, unsigned first, unsigned chunk,
                    const float *wsum, const unsigned long long *cdf, float *grad,
-                   float *loss, unsigned grid, void *stream);
+                   float *loss, float *part, unsigned grid, void *stream);
+unsigned long long rk_guide_part_bytes(void);
 int rk_guide_adam(float *master, float *grad, float *m1, float *m2,
                   void *grid16, void *w16, unsigned step, float lr_grid,
                   float lr_mlp, void *stream);
@@ -60,6 +61,7 @@ void rl_guide_release(struct rl_scene *s)
 		return;
 	dfree((void **)&g->master);
 	dfree((void **)&g->grad);
+	dfree((void **)&g->part);
 	dfree((void **)&g->m1);
 	dfree((void **)&g->m2);
 	dfree(&g->grid16);
@@ -135,6 +137,7 @@ static struct rl_guide *guide_alloc(struct rl_scene *s)
 	hipError_t e;
 	if ((e = hipMalloc((void **)&g->master, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc((void **)&g->grad, GD_NPARAM * sizeof(float))) ||
+	    (e = hipMalloc((void **)&g->part, rk_guide_part_bytes())) ||
 	    (e = hipMalloc((void **)&g->m1, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc((void **)&g->m2, GD_NPARAM * sizeof(float))) ||
 	    (e = hipMalloc(&g->grid16, GD_N_GRID * 2)) ||
@@ -419,8 +422,8 @@ int rl_guide_train(struct rl_scene *s, const struct PtArgs *a, const void *paths
 		/* seed of the step's record sample */
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(a, g->grid16, g->w16, g->master, g->rec, counts, rc,
-		  first, g->chunk, g->scal, cdf, g->grad, g->scal + 1, (unsigned)G.cus * 2u,
-		  st)))
+		  first, g->chunk, g->scal, cdf, g->grad, g->scal + 1, g->part,
+		  (unsigned)G.cus * 2u, st)))
 			return rl_hipfail(r, "guide train");
 		g->step++;
 		if ((r = rk_guide_adam(g->master, g->grad, g->m1, g->m2, g->grid16, g->w16,
@@ -526,7 +529,9 @@ out:
 }
 
 /* Gradient of the mean weighted loss over n records (GdRecord layout with
- * `target` set) into grad (GD_NPARAM floats); loss receives the loss. */
+ * `target` set) into grad (GD_NPARAM floats; the hash-grid part, which the
+ * device accumulates in bf16 pairs, expanded to fp32); loss receives the
+ * loss. */
 int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int n,
                              float *grad, float *loss)
 {
@@ -554,7 +559,7 @@ int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int
 		goto out;
 	}
 	if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, NULL, n, 0, n, dsc,
-	  NULL, g->grad, dsc + 1, (unsigned)((n + 63) / 64), G.stream))) {
+	  NULL, g->grad, dsc + 1, g->part, (unsigned)((n + 15) / 16), G.stream))) {
 		r = rl_hipfail(r, "guide grad");
 		goto out;
 	}
@@ -563,6 +568,17 @@ int rend_hip_guide_grad_host(struct rdata *rd, const void *records, unsigned int
 	    (e = hipMemcpy(sc, dsc, sizeof(sc), hipMemcpyDeviceToHost)) ||
 	    (e = hipMemset(g->grad, 0, GD_NPARAM * sizeof(float))))
 		r = rl_hipfail(e, "guide grad");
+	if (!r) {
+		/* the grid part holds one bf16 pair per entry in its first half:
+		 * expand to fp32 features (from the end, in place) */
+		for (size_t w = GD_N_GRID / 2; w-- > 0;) {
+			uint32_t x;
+			memcpy(&x, &grad[w], 4);
+			uint32_t lo = x << 16, hi = x & 0xffff0000u;
+			memcpy(&grad[2 * w], &lo, 4);
+			memcpy(&grad[2 * w + 1], &hi, 4);
+		}
+	}
 	if (loss)
 		*loss = sc[1];
 out:
@@ -630,7 +646,7 @@ int rend_hip_guide_train_host(struct rdata *rd, const void *records, unsigned in
 		}
 		unsigned first = (g->step + 1u) * 0x9e3779b9u;
 		if ((r = rk_guide_train(&a, g->grid16, g->w16, g->master, drec, dcnt, n, first,
-		  chunk, dsc, cdf, g->grad, dsc + 1, (unsigned)G.cus * 2u, G.stream))) {
+		  chunk, dsc, cdf, g->grad, dsc + 1, g->part, (unsigned)G.cus * 2u, G.stream))) {
 			r = rl_hipfail(r, "guide train");
 			goto out;
 		}

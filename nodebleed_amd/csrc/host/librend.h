@@ -55,6 +55,7 @@ struct rl_pt;     /* pt.c */
 /* Learned proposal state (guide.c). */
 struct rl_guide {
 	float    *master, *grad, *m1, *m2;  /* fp32 parameters, gradient, Adam moments */
+	float    *part;                     /* per-block MLP gradient partial sums */
 	void     *grid16, *w16;             /* fp16 grid / bf16 weight mirrors */
 	float    *mix;                      /* per queued path mixture */
 	void     *rec;                      /* training records of a batch */

@@ -450,12 +450,34 @@ __device__ __forceinline__ int permT_dst(uint32_t i) {
 }
 
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short s16x4;
+typedef short __attribute__((ext_vector_type(2))) s16x2;
+typedef __attribute__((address_space(1))) s16x2 gs16x2;
 
-constexpr int kTrainWaves = 4;
+// Hash-grid gradients: the two features of an entry are one word of two
+// bf16 values at word l * GD_TABLE + idx of the gradient buffer's grid part,
+// added with one packed atomic (half the atomic requests of two fp32 adds;
+// the requests, which the memory-side atomic units serve, bound a training
+// step).  k_guide_adam reads and clears them.
+__device__ __forceinline__ void grid_grad_add(float *grad, size_t entry, float v0, float v1) {
+  s16x2 v;
+  v.x = __builtin_bit_cast(short, (__bf16)v0);
+  v.y = __builtin_bit_cast(short, (__bf16)v1);
+  __builtin_amdgcn_global_atomic_fadd_v2bf16((gs16x2 *)((uint32_t *)grad + entry), v);
+}
+
+// One wave per training block: the hash-grid gradient atomics of a step
+// (the bulk of its time, latency-bound per CU) spread over as many CUs as
+// there are 16-sample tiles, and the block's weight gradients need no LDS
+// atomics.  Each block leaves its weight/bias gradients and loss in its own
+// slot of a partial-sum buffer, summed by k_guide_reduce in a fixed order.
+constexpr int kTrainWaves = 1;
+constexpr uint32_t kTrainMaxBlocks = 512;
 constexpr uint32_t kDwW1 = 0, kDwW2 = GD_HID * GD_IN, kDwW3 = kDwW2 + GD_HID * GD_HID,
                    kDwW4 = kDwW3 + GD_HID * GD_HID, kDwN = kDwW4 + GD_MIX * GD_HID;
 constexpr uint32_t kDbB1 = 0, kDbB2 = GD_HID, kDbB3 = 2 * GD_HID, kDbB4 = 3 * GD_HID,
                    kDbN = 3 * GD_HID + GD_MIX;
+// partial-sum slot of a block: dW, db, loss (floats, padded to 64)
+constexpr uint32_t kPartN = kDwN + kDbN + 1, kPartStride = (kPartN + 63u) & ~63u;
 
 struct TrainLds {
   float dw[kDwN];                                        // weight gradients (fp32)
@@ -503,8 +525,10 @@ __device__ __forceinline__ void put_y(__bf16 *ty, const f32x4 d[4], uint32_t s, 
     for (uint32_t r = 0; r < 4; r++) ty[(16u * t + 4u * g + r) * 16u + s] = (__bf16)d[t][r];
 }
 
-// dW[n][k] += sum over the 16 samples of Y[n][s] X[k][s] (LDS accumulator,
-// row stride ldk; rows n >= nvalid are skipped).
+// dW[n][k] += sum over the 16 samples of Y[n][s] X[k][s] (LDS accumulator
+// of the block's one wave, row stride ldk; rows n >= nvalid are skipped).
+// The lanes of one MFMA result hold distinct (n, k), so plain read-add-write
+// is exact.
 __device__ __forceinline__ void dw_accum(const __bf16 *ty, const __bf16 *tx, int nrows,
                                          int nvalid, int krows, float *dw, int ldk) {
   const uint32_t l = lane_id(), c = l & 15u, g = l >> 4;
@@ -517,7 +541,7 @@ __device__ __forceinline__ void dw_accum(const __bf16 *ty, const __bf16 *tx, int
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int n = nt + 4 * (int)g + r;
-        if (n < nvalid && d[r] != 0.0f) atomicAdd(&dw[n * ldk + kt + (int)c], d[r]);
+        if (n < nvalid) dw[n * ldk + kt + (int)c] += d[r];
       }
     }
   }
@@ -561,7 +585,7 @@ __device__ __forceinline__ void back64(const __bf16 *__restrict__ WT, const bf16
 // the mean of -log q(dir) (records drawn in proportion to target) into grad;
 // q is the vMF mixture p, or with a.train_bsdf = b > 0 the density
 // (1 - b) p + b cos/pi that one-sample MIS with the BSDF samples.
-__global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__restrict__ grid,
+__global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__restrict__ grid,
                                                      const __bf16 *__restrict__ wp,
                                                      const __bf16 *__restrict__ wt,
                                                      const float *__restrict__ master,
@@ -572,7 +596,7 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
                                                      const float *__restrict__ wsum,
                                                      const uint64_t *__restrict__ cdf,
                                                      float *__restrict__ grad,
-                                                     float *__restrict__ loss) {
+                                                     float *__restrict__ part) {
   __shared__ TrainLds S;
   for (uint32_t k = threadIdx.x; k < kDwN; k += blockDim.x) S.dw[k] = 0.0f;
   for (uint32_t k = threadIdx.x; k < kDbN; k += blockDim.x) S.db[k] = 0.0f;
@@ -753,7 +777,7 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
 #pragma unroll
     for (int sl = 0; sl < 10; sl++) {
       const float v = sum16(go[sl]);
-      if (s == 0 && v != 0.0f) atomicAdd(&S.db[kDbB4 + 10u * g + (uint32_t)sl], v);
+      if (s == 0) S.db[kDbB4 + 10u * g + (uint32_t)sl] += v;
     }
     bf16x8 yb[2];
 #pragma unroll
@@ -780,7 +804,7 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
           const float v = sum16(d[t][rr]);
-          if (s == 0 && v != 0.0f) atomicAdd(&S.db[bo + 16u * t + 4u * g + rr], v);
+          if (s == 0) S.db[bo + 16u * t + 4u * g + rr] += v;
         }
       yb[0] = pack_b(d[0], d[1]);
       yb[1] = pack_b(d[2], d[3]);
@@ -799,7 +823,7 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
         const float v = sum16(d[t][rr]);
-        if (s == 0 && v != 0.0f) atomicAdd(&S.db[kDbB1 + 16u * t + 4u * g + rr], v);
+        if (s == 0) S.db[kDbB1 + 16u * t + 4u * g + rr] += v;
       }
     yb[0] = pack_b(d[0], d[1]);
     yb[1] = pack_b(d[2], d[3]);
@@ -836,44 +860,68 @@ __global__ __launch_bounds__(256) void k_guide_train(PtArgs a, const __half *__r
             if (s == 0 && (v0 != 0.0f || v1 != 0.0f)) {
               // lane 16 g holds a valid sample's corner unless the whole
               // group is invalid (then v0 = v1 = 0)
-              const size_t go2 = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
-              atomicAdd(&grad[go2], v0);
-              atomicAdd(&grad[go2 + 1], v1);
+              grid_grad_add(grad, (size_t)l * GD_TABLE + idx, v0, v1);
             }
           } else if (d0 != 0.0f || d1 != 0.0f) {
-            const size_t go2 = ((size_t)l * GD_TABLE + idx) * GD_FEAT;
-            atomicAdd(&grad[go2], w * d0);
-            atomicAdd(&grad[go2 + 1], w * d1);
+            grid_grad_add(grad, (size_t)l * GD_TABLE + idx, w * d0, w * d1);
           }
         }
       }
   }
-  if (loss) {
-    for (int o2 = 32; o2 > 0; o2 >>= 1) lacc += __shfl_down(lacc, o2);
-    if (lane == 0 && lacc != 0.0f) atomicAdd(loss, lacc);
-  }
+  for (int o2 = 32; o2 > 0; o2 >>= 1) lacc += __shfl_down(lacc, o2);
   __syncthreads();
-  // flush the block's weight and bias gradients
-  for (uint32_t k = threadIdx.x; k < kDwN; k += blockDim.x) {
-    const float v = S.dw[k];
-    if (v == 0.0f) continue;
-    uint32_t dst;
-    if (k < kDwW2) dst = GD_OFS_W1 + k;
-    else if (k < kDwW3) dst = GD_OFS_W2 + (k - kDwW2);
-    else if (k < kDwW4) dst = GD_OFS_W3 + (k - kDwW3);
-    else dst = GD_OFS_W4 + (k - kDwW4);
-    atomicAdd(&grad[dst], v);
+  // the block's weight and bias gradients and loss into its partial slot
+  float *__restrict__ slot = part + (size_t)blockIdx.x * kPartStride;
+  for (uint32_t k = threadIdx.x; k < kDwN + kDbN; k += blockDim.x)
+    slot[k] = k < kDwN ? S.dw[k] : S.db[k - kDwN];
+  if (threadIdx.x == 0) slot[kDwN + kDbN] = lacc;
+}
+
+// Sums the nb training blocks' partial slots (block order, so the MLP
+// gradients of a step do not depend on scheduling) into the MLP part of
+// grad, which it overwrites, and adds the loss to *loss.
+// A 256-thread block sums 16 entries: 16 groups of 16 threads each add
+// every 16th slot, then the group sums are added in group order.
+__global__ __launch_bounds__(256) void k_guide_reduce(const float *__restrict__ part, uint32_t nb,
+                                                      float *__restrict__ grad,
+                                                      float *__restrict__ loss) {
+  __shared__ float acc[16][17];
+  const uint32_t c = threadIdx.x & 15u, grp = threadIdx.x >> 4;
+  const uint32_t k = blockIdx.x * 16u + c;
+  float v = 0.0f;
+  if (k < kPartN) {
+    uint32_t b = grp;
+    for (; b + 48u < nb; b += 64u) {
+      const float v0 = part[(size_t)b * kPartStride + k];
+      const float v1 = part[(size_t)(b + 16u) * kPartStride + k];
+      const float v2 = part[(size_t)(b + 32u) * kPartStride + k];
+      const float v3 = part[(size_t)(b + 48u) * kPartStride + k];
+      v += v0;
+      v += v1;
+      v += v2;
+      v += v3;
+    }
+    for (; b < nb; b += 16u) v += part[(size_t)b * kPartStride + k];
   }
-  for (uint32_t k = threadIdx.x; k < kDbN; k += blockDim.x) {
-    const float v = S.db[k];
-    if (v == 0.0f) continue;
-    uint32_t dst;
-    if (k < kDbB2) dst = GD_OFS_B1 + k;
-    else if (k < kDbB3) dst = GD_OFS_B2 + (k - kDbB2);
-    else if (k < kDbB4) dst = GD_OFS_B3 + (k - kDbB3);
-    else dst = GD_OFS_B4 + (k - kDbB4);
-    atomicAdd(&grad[dst], v);
+  acc[grp][c] = v;
+  __syncthreads();
+  if (grp != 0 || k >= kPartN) return;
+  v = 0.0f;
+  for (uint32_t j = 0; j < 16u; j++) v += acc[j][c];
+  if (k == kDwN + kDbN) {
+    if (loss) loss[0] += v;
+    return;
   }
+  uint32_t dst;
+  if (k < kDwW2) dst = GD_OFS_W1 + k;
+  else if (k < kDwW3) dst = GD_OFS_W2 + (k - kDwW2);
+  else if (k < kDwW4) dst = GD_OFS_W3 + (k - kDwW3);
+  else if (k < kDwN) dst = GD_OFS_W4 + (k - kDwW4);
+  else if (k < kDwN + kDbB2) dst = GD_OFS_B1 + (k - kDwN);
+  else if (k < kDwN + kDbB3) dst = GD_OFS_B2 + (k - kDwN - kDbB2);
+  else if (k < kDwN + kDbB4) dst = GD_OFS_B3 + (k - kDwN - kDbB3);
+  else dst = GD_OFS_B4 + (k - kDwN - kDbB4);
+  grad[dst] = v;
 }
 
 // Mixture records (GD_MIX floats) of queued hit points (pn == nullptr) or of
@@ -1008,29 +1056,47 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
 }
 
 // Adam on the fp32 master parameters; refreshes the fp16/bf16 mirrors and
-// clears the gradient.  Hash-grid entries a step did not touch (gradient
-// exactly zero) keep their parameters and moments: their master copy and
-// fp16 mirror are neither read nor written, so a step costs one read of the
-// grid gradient plus the touched entries.
+// clears the gradient.  The grid part of the gradient holds one bf16 pair
+// per entry (grid_grad_add) in its first GD_N_GRID / 2 words; the MLP part
+// is fp32.  Hash-grid features a step did not touch (gradient exactly zero)
+// keep their parameters and moments: their master copy and fp16 mirror are
+// neither read nor written, so a step costs one read of the packed grid
+// gradient plus the touched entries.
+__device__ __forceinline__ float adam_one(float *master, float *m1, float *m2, uint32_t i,
+                                          float g, float lr, float c1, float c2) {
+  const float b1 = 0.9f, b2 = 0.99f, eps = 1e-10f;
+  const float a = b1 * m1[i] + (1.0f - b1) * g;
+  const float b = b2 * m2[i] + (1.0f - b2) * g * g;
+  m1[i] = a;
+  m2[i] = b;
+  const float p = master[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  master[i] = p;
+  return p;
+}
+
 __global__ void k_guide_adam(float *__restrict__ master, float *__restrict__ grad,
                              float *__restrict__ m1, float *__restrict__ m2,
                              __half *__restrict__ grid16, __bf16 *__restrict__ w16,
                              uint32_t step, float lr_grid, float lr_mlp) {
-  const float b1 = 0.9f, b2 = 0.99f, eps = 1e-10f;
-  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < GD_NPARAM;
+  const float c1 = 1.0f - powf(0.9f, (float)step), c2 = 1.0f - powf(0.99f, (float)step);
+  constexpr uint32_t kWords = GD_N_GRID / 2u;
+  uint32_t *__restrict__ gw = (uint32_t *)grad;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kWords + GD_NMLP;
        i += gridDim.x * blockDim.x) {
-    const float g = grad[i];
-    if (g != 0.0f || i >= GD_N_GRID) {
-      const float a = b1 * m1[i] + (1.0f - b1) * g;
-      const float b = b2 * m2[i] + (1.0f - b2) * g * g;
-      m1[i] = a;
-      m2[i] = b;
-      const float lr = i < GD_N_GRID ? lr_grid : lr_mlp;
-      const float p = master[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
-      master[i] = p;
-      grad[i] = 0.0f;
-      if (i < GD_N_GRID) grid16[i] = __float2half(p);
+    if (i < kWords) {
+      const uint32_t w = gw[i];
+      if (!w) continue;
+      gw[i] = 0u;
+      const float g0 = __uint_as_float(w << 16), g1 = __uint_as_float(w & 0xffff0000u);
+      if (g0 != 0.0f)
+        grid16[2u * i] = __float2half(adam_one(master, m1, m2, 2u * i, g0, lr_grid, c1, c2));
+      if (g1 != 0.0f)
+        grid16[2u * i + 1u] =
+            __float2half(adam_one(master, m1, m2, 2u * i + 1u, g1, lr_grid, c1, c2));
+    } else {
+      const uint32_t p = GD_N_GRID + (i - kWords);
+      adam_one(master, m1, m2, p, grad[p], lr_mlp, c1, c2);
+      grad[p] = 0.0f;
     }
   }
   // bf16 mirror of the weights (not biases), packed W1 W2 W3 W4
@@ -1159,19 +1225,27 @@ extern "C" int rk_guide_cdf(const float *dense, const float *wsum, unsigned long
 }
 
 // `grid` bounds the number of 256-thread blocks (each holds 69 KB of LDS).
+extern "C" unsigned long long rk_guide_part_bytes(void) {
+  return (unsigned long long)kTrainMaxBlocks * kPartStride * sizeof(float);
+}
+
+// part: rk_guide_part_bytes() of device memory.
 extern "C" int rk_guide_train(const PtArgs *a, const void *grid16, const void *w16,
                               const float *master, const void *rec, const unsigned *counts,
                               unsigned cap, unsigned first, unsigned chunk, const float *wsum,
                               const unsigned long long *cdf, float *grad, float *loss,
-                              unsigned grid, void *stream) {
+                              float *part, unsigned grid, void *stream) {
   if (!chunk) return 0;
   const unsigned need = (chunk + 16u * kTrainWaves - 1) / (16u * kTrainWaves);
-  const unsigned nb = grid < need ? (grid ? grid : 1u) : need;
+  unsigned nb = grid < need ? (grid ? grid : 1u) : need;
+  if (nb > kTrainMaxBlocks) nb = kTrainMaxBlocks;
   const __bf16 *w = (const __bf16 *)w16;
   hipLaunchKernelGGL(k_guide_train, dim3(nb), dim3(64 * kTrainWaves), 0, (hipStream_t)stream,
                      *a, (const __half *)grid16, w + GD_NW16, w + 2 * GD_NW16, master,
                      (const GdRecord *)rec, counts, cap, first, chunk, wsum,
-                     (const uint64_t *)cdf, grad, loss);
+                     (const uint64_t *)cdf, grad, part);
+  hipLaunchKernelGGL(k_guide_reduce, dim3((kPartN + 15u) / 16u), dim3(256), 0,
+                     (hipStream_t)stream, part, nb, grad, loss);
   return (int)hipGetLastError();
 }
 
