@@ -268,6 +268,72 @@ def bench_tlas(a):
     return 0
 
 
+def bench_blas(a):
+    """Scene load of the 1M-triangle terrain (rs_commit: rend_prepstatic,
+    rend_prepdynamic; then the upload) with the reference's host SAH BLAS
+    build (build_bvh8, repack, upload) against the device LBVH build, and
+    the closest-hit throughput of 1920x1080 diffuse rays through either
+    tree.  Step = one traversal launch (the build is timed once per tree)."""
+    from nodebleed_amd import REND_BLAS_GPU, REND_BLAS_HOST, RsView
+    L = lib()
+    n = W * H
+    res = {}
+    try:
+        for name, mode in (("host", REND_BLAS_HOST), ("device", REND_BLAS_GPU)):
+            check(L.rend_hip_set_blas_build(mode), "blas mode")
+            L.rend_staticinit(64, 32)
+            rs = L.rs_new()
+            check(L.rs_config(rs, 2, a.seed, 1.0), "scene")
+            scene = Scene(None, builder=rs, width=W, height=H, spp=1)
+            # time a fresh commit of the same builder: rend_prepstatic +
+            # rend_prepdynamic, then the upload (and, for device, the build)
+            rd = L.rs_rdata_new()
+            check(L.rs_commit(rs, rd, W, H, 1), "warm commit")
+            check(L.rend_hip_sync_scene(rd), "warm upload")
+            L.rend_release(rd)
+            L.rs_rdata_free(rd)
+            rd = L.rs_rdata_new()
+            t = time.perf_counter()
+            check(L.rs_commit(rs, rd, W, H, 1), "commit")
+            tc = time.perf_counter() - t
+            check(L.rend_hip_sync_scene(rd), "upload")
+            tl = time.perf_counter() - t
+            L.rend_release(rd)
+            L.rs_rdata_free(rd)
+            bufs = [L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize),
+                    L.rend_hip_malloc(n * RAY.itemsize), L.rend_hip_malloc(n * HIT.itemsize)]
+            d_prim, d_phit, d_diff, d_dhit = bufs
+            stream = L.rend_hip_stream()
+            check(L.rend_hip_gen_camrays(scene.rd, d_prim, W, H, 1, a.seed, stream), "camrays")
+            check(L.rend_hip_trace(scene.rd, d_prim, d_phit, n, stream, None), "primary")
+            check(L.rend_hip_gen_diffuse(scene.rd, d_prim, d_phit, d_diff, n, a.seed, stream),
+                  "diffuse rays")
+            st = TraceStats()
+            check(L.rend_hip_trace(scene.rd, d_diff, d_dhit, n, stream, C.byref(st)), "count")
+            timed_launches(L, scene, d_diff, d_dhit, n, max(1, a.warmup))
+            tr = timed_launches(L, scene, d_diff, d_dhit, n, a.steps) / a.steps
+            res[name] = {"commit_s": round(tc, 4), "load_s": round(tl, 4),
+                         "diffuse_trace_ms": round(tr, 4),
+                         "diffuse_mrays_s": round(n / tr / 1e3, 1),
+                         "visits_per_ray": {"nodes": round(st.nodes / n, 3),
+                                            "leaves": round(st.leaves / n, 3)}}
+            log(f"{name} BLAS: commit {tc * 1e3:.1f} ms, commit + upload {tl * 1e3:.1f} ms, "
+                f"diffuse trace {tr:.3f} ms, nodes/ray {st.nodes / n:.2f}")
+            for p in bufs:
+                L.rend_hip_free(p)
+            scene.close()
+    finally:
+        L.rend_hip_set_blas_build(REND_BLAS_HOST)
+    dev = res["device"]["load_s"] * 1e3
+    emit("scene load ms (rs_commit + upload), device BLAS build", dev, "ms", a.steps, a.warmup,
+         res["device"]["diffuse_trace_ms"],
+         {"workload": "1M-triangle displaced terrain: BLAS build + TLAS + upload, then "
+                      "1920x1080 diffuse rays traced through the tree",
+          "host": res["host"], "device": res["device"],
+          "speedup": round(res["host"]["load_s"] * 1e3 / dev, 1)}, higher=False)
+    return 0
+
+
 def uneven_views(seed=1, n=32):
     """n cameras over the terrain: resolutions from {320x240 .. 3840x2160},
     spp uniform in 4..64, eyes on a ring above the terrain."""
@@ -476,7 +542,8 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after timing, run one plain copy kernel of MIB MiB "
                          "(reference for hardware byte counters)")
-    ap.add_argument("--mode", default="trace", choices=("trace", "pt", "views", "guide", "tlas"),
+    ap.add_argument("--mode", default="trace",
+                    choices=("trace", "pt", "views", "guide", "tlas", "blas"),
                     help="trace: closest-hit microbench (the headline line); pt: "
                          "instanced path tracing; views: 32 uneven cameras in one batched "
                          "launch; guide: equal-time guided vs plain path tracing; tlas: "
@@ -514,7 +581,7 @@ def main():
         check(L.rend_hip_pt_graphs(0), "graphs")
     if a.mode != "trace":
         sys.exit({"pt": bench_pt, "views": bench_views, "guide": bench_guide,
-                  "tlas": bench_tlas}[a.mode](a))
+                  "tlas": bench_tlas, "blas": bench_blas}[a.mode](a))
 
     t0 = time.perf_counter()
     scene = Scene(config=2, seed=a.seed, detail=1.0, width=W, height=H, spp=1)
@@ -551,10 +618,10 @@ def main():
     if a.sweep:
         tun0 = (C.c_uint * 3)()
         check(L.rend_hip_get_tuning(tun0), "tuning")
-        for variant in range(2):
+        for variant in range(4):
             check(L.rend_hip_set_trace_variant(variant), "variant")
-            for mult in (0, 8):
-                for refill in (16, 32, 48, 56, 64):
+            for mult in (0,):
+                for refill in (32, 48, 56):
                     check(L.rend_hip_set_tuning(mult, refill), "tuning")
                     ms = timed_launches(L, scene, d_diff, d_dhit, n, 5) / 5
                     msp = timed_launches(L, scene, d_prim, d_phit, n, 5) / 5

@@ -60,6 +60,28 @@ int          rend_hip_set_rcp(int mode);
 #define REND_TLAS_GPU   1
 int          rend_hip_set_tlas_build(int mode);
 
+/* Where the BLASes of scenes prepared by later rend_prepstatic calls are
+ * built (rend_prepstatic, rend.c:3033-3071; build_bvh8, rend.c:458-709).
+ * HOST (default): the reference's SAH-binned build, byte for byte, into
+ * rd->bnodes, repacked and uploaded.  GPU: a device LBVH build per mesh
+ * (Morton order, radix tree, collapsed to 8-wide nodes with leaves of at
+ * most 4 triangles) at the next upload, from the triangles rend_prepstatic
+ * reads; rd->bnodes then holds, per mesh, only a root whose first child box
+ * is the mesh's bounds (what upd_rinsts reads, main.c:117-157).  The leaves
+ * carry the reference's FP32 triangle data, so hits are the same up to
+ * which of two triangles at exactly equal distance is reported; visit
+ * counts are not. */
+#define REND_BLAS_HOST  0
+#define REND_BLAS_GPU   1
+int          rend_hip_set_blas_build(int mode);
+
+/* The device BLAS of the mesh at triangle offset triofs as the traversal
+ * reads it (after any pending upload or device build): bytes receives its
+ * size, dst (cap bytes, may be NULL) its nodes (struct dnode) and leaves
+ * (struct dleaf) in the arena layout of dlayout.h, the root first. */
+int          rend_hip_blas_download(struct rdata *rd, unsigned int triofs, void *dst,
+                                    unsigned long long cap, unsigned long long *bytes);
+
 /* Re-upload the scene of rd (normally done by rend_prepstatic and
  * rend_prepdynamic). */
 int          rend_hip_upload(struct rdata *rd);

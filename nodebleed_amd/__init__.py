@@ -37,6 +37,8 @@ REND_RCP_IEEE = 1
 REND_RCP_XEON_TABLE = 2
 REND_TLAS_HOST = 0
 REND_TLAS_GPU = 1
+REND_BLAS_HOST = 0
+REND_BLAS_GPU = 1
 INST_ID_BITS = 12
 NODE_LEAF = 0x80000000
 
@@ -76,6 +78,9 @@ _SIGS = {
     "rend_hip_devinfo": (C.c_int, [C.POINTER(C.c_int), C.c_char_p, C.c_uint]),
     "rend_hip_set_rcp": (C.c_int, [C.c_int]),
     "rend_hip_set_tlas_build": (C.c_int, [C.c_int]),
+    "rend_hip_set_blas_build": (C.c_int, [C.c_int]),
+    "rend_hip_blas_download": (C.c_int, [C.c_void_p, C.c_uint, C.c_void_p, C.c_ulonglong,
+                                         C.POINTER(C.c_ulonglong)]),
     "rend_hip_sync_scene": (C.c_int, [C.c_void_p]),
     "rend_hip_tlas_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_ulonglong,
                                          C.POINTER(C.c_uint)]),

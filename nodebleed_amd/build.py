@@ -31,7 +31,8 @@ ARCH = "gfx950"
 
 HOST_SRC = ["host/bvh8.c", "host/rend.c", "host/frame.c", "host/pt.c",
             "host/scenegen.c", "host/guide.c", "host/approx.c", "host/gltf.c"]
-HIP_SRC = ["kern/trace.hip", "kern/frame.hip", "kern/pt.hip", "kern/guide.hip", "kern/tlas.hip"]
+HIP_SRC = ["kern/trace.hip", "kern/frame.hip", "kern/pt.hip", "kern/guide.hip", "kern/tlas.hip",
+           "kern/blas.hip"]
 
 CFLAGS = ["-std=c11", "-O2", "-fPIC", "-ffp-contract=off", "-fno-strict-aliasing",
           "-mavx2", "-mfma",

@@ -31,6 +31,7 @@ struct rl_global {
 	int                 variant;     /* traversal occupancy variant */
 	unsigned            step_batch;  /* lanes a leaf/setup step waits for, 0 = default */
 	int                 tlas_gpu;    /* rend_prepdynamic builds the TLAS on the device */
+	int                 blas_gpu;    /* rend_prepstatic leaves the BLAS build to the device */
 	unsigned           *ctl;         /* RL_CTL_SLOTS x RK_CTL_WORDS, then RK_DIAG_WORDS */
 	unsigned           *diag;        /* error block shared by all traversal launches */
 	unsigned            ctl_next;
@@ -88,6 +89,7 @@ struct rl_scene {
 	unsigned             nblas;
 	unsigned            *blas_triofs;
 	unsigned long long  *blas_dev;   /* byte offset of each BLAS root */
+	unsigned long long  *blas_len;   /* its bytes in the arena */
 	void                *insts;      /* device dinst[] */
 	size_t               insts_cap;
 	unsigned             ninsts;
@@ -100,6 +102,7 @@ struct rl_scene {
 	int                  static_ok, dynamic_ok;
 	int                  static_dirty, dynamic_dirty, tlas_built;
 	int                  tlas_dev;   /* the pending TLAS is built on the device */
+	int                  blas_dev_build; /* BLASes are built on the device at upload */
 	void                *aabbs_dev;  /* instance world AABBs for the device build */
 	unsigned            *tlas_info;  /* device build result: nodes, error */
 	size_t               aabbs_cap;

@@ -167,6 +167,14 @@ int rend_hip_set_tlas_build(int mode)
 	return 0;
 }
 
+int rend_hip_set_blas_build(int mode)
+{
+	if (mode != REND_BLAS_HOST && mode != REND_BLAS_GPU)
+		return rl_fail(REND_EINVAL, "BLAS build mode %d", mode);
+	G.blas_gpu = mode == REND_BLAS_GPU;
+	return 0;
+}
+
 int rend_hip_set_tuning(unsigned int grid_mult, unsigned int refill)
 {
 	if (refill < 1 || refill > 64)
@@ -328,6 +336,7 @@ static void scene_del(struct rdata *rd)
 			rl_pt_release(s);
 			free(s->blas_triofs);
 			free(s->blas_dev);
+			free(s->blas_len);
 			free(s);
 			g_scenes[i] = NULL;
 		}
@@ -458,6 +467,82 @@ static int arena_reserve(struct rl_scene *s, size_t bytes, size_t keep)
 	return 0;
 }
 
+unsigned long long rk_blas_ws_bytes(unsigned n);
+int rk_blas_build(const void *tris, unsigned n, void *ws, unsigned counts[2], void *stream);
+int rk_blas_reloc(const void *ws, unsigned n, const unsigned counts[2], void *out,
+                  void *stream);
+
+static int upload_tris(struct rdata *rd, struct rl_scene *s)
+{
+	hipError_t e;
+	size_t tb = (size_t)s->maxtris * sizeof(struct rtri);
+	if (tb > s->tris_cap) {
+		dfree((void **)&s->tris);
+		if ((e = hipMalloc((void **)&s->tris, tb)) != hipSuccess)
+			return rl_hipfail(e, "triangles");
+		s->tris_cap = tb;
+	}
+	if (tb && (e = hipMemcpy(s->tris, rd->tris, tb, hipMemcpyHostToDevice)) != hipSuccess)
+		return rl_hipfail(e, "triangle upload");
+	return 0;
+}
+
+/* Every distinct BLAS built on the device from the uploaded triangles
+ * (kern/blas.hip), placed one after another from the arena's start. */
+static int build_blas_dev(struct rdata *rd, struct rl_scene *s, size_t *total_out)
+{
+	int r = upload_tris(rd, s);
+	if (r)
+		return r;
+	unsigned maxn = 0;
+	for (unsigned b = 0; b < s->nblas; b++)
+		for (unsigned j = 0; j < rd->instcnt; j++)
+			if (rd->insts[j].triofs == s->blas_triofs[b] && rd->insts[j].tricnt > maxn)
+				maxn = rd->insts[j].tricnt;
+	void *ws = NULL;
+	hipError_t e = hipMalloc(&ws, (size_t)rk_blas_ws_bytes(maxn));
+	if (e != hipSuccess)
+		return rl_hipfail(e, "BLAS build workspace");
+	size_t total = 0;
+	for (unsigned b = 0; b < s->nblas && !r; b++) {
+		unsigned triofs = s->blas_triofs[b], tricnt = 0;
+		for (unsigned j = 0; j < rd->instcnt; j++)
+			if (rd->insts[j].triofs == triofs) {
+				tricnt = rd->insts[j].tricnt;
+				break;
+			}
+		unsigned cnt[2] = {0, 0};
+		const unsigned char *tp = (const unsigned char *)s->tris +
+		  (size_t)triofs * sizeof(struct rtri);
+		int k = rk_blas_build(tp, tricnt, ws, cnt, G.stream);
+		if (k) {
+			r = k < 0 ? rl_fail(REND_EINVAL, "device BLAS build overflow (%u tris)", tricnt)
+			          : rl_hipfail(k, "device BLAS build");
+			break;
+		}
+		size_t sz = (size_t)cnt[0] * sizeof(struct dnode) + (size_t)cnt[1] * sizeof(struct dleaf);
+		if ((size_t)cnt[0] * sizeof(struct dnode) + (size_t)cnt[1] * sizeof(struct dleaf) >=
+		    ((size_t)1 << 31)) {
+			r = rl_fail(REND_EINVAL, "device BLAS of %u tris exceeds 2 GB", tricnt);
+			break;
+		}
+		if ((r = arena_reserve(s, total + sz, total)))
+			break;
+		if ((k = rk_blas_reloc(ws, tricnt, cnt, s->arena + total, G.stream))) {
+			r = rl_hipfail(k, "device BLAS placement");
+			break;
+		}
+		s->blas_dev[b] = total;
+		s->blas_len[b] = sz;
+		total += (sz + 255) & ~(size_t)255;
+	}
+	if (!r && (e = hipStreamSynchronize(G.stream)) != hipSuccess)
+		r = rl_hipfail(e, "device BLAS build");
+	hipFree(ws);
+	*total_out = total;
+	return r;
+}
+
 static int upload_static(struct rdata *rd, struct rl_scene *s)
 {
 	int r = rl_device();
@@ -466,13 +551,28 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 	/* Distinct BLAS roots referenced by instances. */
 	free(s->blas_triofs);
 	free(s->blas_dev);
+	free(s->blas_len);
 	s->blas_triofs = malloc(((size_t)rd->instcnt + 1) * sizeof(unsigned));
 	s->blas_dev = malloc(((size_t)rd->instcnt + 1) * sizeof(unsigned long long));
+	s->blas_len = malloc(((size_t)rd->instcnt + 1) * sizeof(unsigned long long));
 	s->nblas = 0;
-	if (!s->blas_triofs || !s->blas_dev)
+	if (!s->blas_triofs || !s->blas_dev || !s->blas_len)
 		return rl_fail(REND_ENOMEM, "blas map");
 
 	size_t total = 0;
+	if (s->blas_dev_build) {
+		for (unsigned j = 0; j < rd->instcnt; j++)
+			if (blas_find(s, rd->insts[j].triofs) < 0)
+				s->blas_triofs[s->nblas++] = rd->insts[j].triofs;
+		if ((r = build_blas_dev(rd, s, &total)))
+			return r;
+		size_t tlas_res = 2 * (size_t)(s->maxinsts ? s->maxinsts : 1) *
+		  sizeof(struct bnode8);
+		if ((r = arena_reserve(s, total + tlas_res, total)))
+			return r;
+		s->blas_bytes = total;
+		goto shading;
+	}
 	long long *sizes = malloc(((size_t)rd->instcnt + 1) * sizeof(long long));
 	if (!sizes)
 		return rl_fail(REND_ENOMEM, "blas sizes");
@@ -492,6 +592,7 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 		}
 		s->blas_triofs[s->nblas] = ri->triofs;
 		s->blas_dev[s->nblas] = total;
+		s->blas_len[s->nblas] = (unsigned long long)sz;
 		sizes[s->nblas] = sz;
 		s->nblas++;
 		total += ((size_t)sz + 255) & ~(size_t)255;
@@ -543,6 +644,7 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 	free(sizes);
 	s->blas_bytes = total;
 
+shading:
 	/* Normals and materials for shading. */
 	size_t nb = (size_t)s->maxtris * sizeof(struct rnrm);
 	size_t mb = (size_t)s->maxmtls * sizeof(struct rmtl);
@@ -559,16 +661,9 @@ static int upload_static(struct rdata *rd, struct rl_scene *s)
 			return rl_hipfail(e, "materials");
 		s->mtls_cap = mb;
 	}
-	size_t tb = (size_t)s->maxtris * sizeof(struct rtri);
-	if (tb > s->tris_cap) {
-		dfree((void **)&s->tris);
-		if ((e = hipMalloc((void **)&s->tris, tb)) != hipSuccess)
-			return rl_hipfail(e, "triangles");
-		s->tris_cap = tb;
-	}
-	if ((tb && (e = hipMemcpy(s->tris, rd->tris, tb, hipMemcpyHostToDevice))
-	      != hipSuccess) ||
-	    (nb && (e = hipMemcpy(s->nrms, rd->nrms, nb, hipMemcpyHostToDevice))
+	if (!s->blas_dev_build && (r = upload_tris(rd, s)))
+		return r;
+	if ((nb && (e = hipMemcpy(s->nrms, rd->nrms, nb, hipMemcpyHostToDevice))
 	      != hipSuccess) ||
 	    (mb && (e = hipMemcpy(s->mtls, rd->mtls, mb, hipMemcpyHostToDevice))
 	      != hipSuccess))
@@ -737,8 +832,26 @@ static int upload_dynamic(struct rdata *rd, struct rl_scene *s)
 	return 0;
 }
 
+/* GPU BLAS mode: the host arena keeps, per mesh, a root whose first child
+ * box is the mesh's bounds and whose first child word is non-zero (the
+ * reference's "already built" test, rend.c:3039); upd_rinsts reads the root's
+ * bounds (main.c:117-157).  The tree itself is built on the device. */
+static void stub_root(struct bnode8 *rn, struct vec3 lo, struct vec3 hi)
+{
+	memset(rn, 0, sizeof(*rn));
+	float *b[6] = {(float *)&rn->minx8, (float *)&rn->maxx8, (float *)&rn->miny8,
+	  (float *)&rn->maxy8, (float *)&rn->minz8, (float *)&rn->maxz8};
+	const float v[6] = {lo.x, hi.x, lo.y, hi.y, lo.z, hi.z};
+	for (int a = 0; a < 6; a++)
+		for (int c = 0; c < 8; c++)
+			b[a][c] = c ? ((a & 1) ? -FLT_MAX : FLT_MAX) : v[a];
+	const unsigned w = DNODE_LEAF;
+	memcpy(&rn->children8, &w, sizeof(w));
+}
+
 void rend_prepstatic(struct rdata *rd)
 {
+	const int dev_build = G.blas_gpu;
 	for (unsigned j = 0; j < rd->instcnt; j++) {
 		struct rinst *ri = &rd->insts[j];
 		unsigned triofs = ri->triofs;
@@ -747,6 +860,16 @@ void rend_prepstatic(struct rdata *rd)
 			continue;  /* already built (rend.c:3039) */
 		unsigned tricnt = ri->tricnt;
 		const struct rtri *tp = &rd->tris[triofs];
+		if (dev_build) {
+			struct vec3 lo = {FLT_MAX, FLT_MAX, FLT_MAX};
+			struct vec3 hi = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+			for (unsigned i = 0; i < tricnt; i++) {
+				lo = rv_min(rv_min(rv_min(lo, tp[i].v0), tp[i].v1), tp[i].v2);
+				hi = rv_max(rv_max(rv_max(hi, tp[i].v0), tp[i].v1), tp[i].v2);
+			}
+			stub_root(rn, lo, hi);
+			continue;
+		}
 		unsigned *imap = malloc(((size_t)tricnt + 1) * sizeof(*imap));
 		struct aabb *ab = malloc(((size_t)tricnt + 1) * sizeof(*ab));
 		if (!imap || !ab) {
@@ -780,6 +903,7 @@ void rend_prepstatic(struct rdata *rd)
 		rl_fail(REND_ENOMEM, "scene registry full");
 		return;
 	}
+	s->blas_dev_build = dev_build;
 	/* Device upload happens at the first trace or frame after this. */
 	s->static_dirty = 1;
 	if (s->tlas_built)
@@ -948,6 +1072,29 @@ int rend_hip_tlas_download(struct rdata *rd, void *dst, unsigned long long cap,
 	if (!e)
 		e = hipStreamSynchronize(G.stream);
 	return e ? rl_hipfail(e, "TLAS download") : 0;
+}
+
+int rend_hip_blas_download(struct rdata *rd, unsigned int triofs, void *dst,
+                           unsigned long long cap, unsigned long long *bytes)
+{
+	struct rl_scene *s;
+	int r = check_ready(rd, &s);
+	if (r)
+		return r;
+	int b = blas_find(s, triofs);
+	if (b < 0)
+		return rl_fail(REND_EINVAL, "no BLAS at triangle offset %u", triofs);
+	if (bytes)
+		*bytes = s->blas_len[b];
+	if (!dst)
+		return 0;
+	if (cap < s->blas_len[b])
+		return rl_fail(REND_EINVAL, "BLAS is %llu bytes, buffer %llu", s->blas_len[b], cap);
+	hipError_t e = hipMemcpyAsync(dst, s->arena + s->blas_dev[b], s->blas_len[b],
+	  hipMemcpyDeviceToHost, G.stream);
+	if (!e)
+		e = hipStreamSynchronize(G.stream);
+	return e ? rl_hipfail(e, "BLAS download") : 0;
 }
 
 int rend_hip_sync_scene(struct rdata *rd)

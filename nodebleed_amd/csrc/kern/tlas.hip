@@ -29,8 +29,14 @@
 #include <stdint.h>
 
 #include "../dlayout.h"
+#include "bvh_dev.h"
 
 namespace {
+
+using bvhdev::Box;
+using bvhdev::half_area;
+using bvhdev::spread10;
+using bvhdev::write_node;
 
 constexpr uint32_t kMax = 4096;       // instances (12-bit instance ids)
 constexpr uint32_t kThreads = 1024;
@@ -50,24 +56,11 @@ struct BuildLds {
   uint32_t cnt, maxd, err;
 };
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
 // Longest common prefix of sorted keys i and j, -1 outside [0, n).
 __device__ __forceinline__ int delta(const uint64_t *key, int n, int i, int j) {
   if (j < 0 || j >= n) return -1;
   return __clzll(key[i] ^ key[j]);
 }
-
-struct Box {
-  float lo[3], hi[3];
-};
 
 __device__ __forceinline__ Box leaf_box(const float4 *ab, const uint16_t *inst, uint32_t pos) {
   const uint32_t k = inst[pos];
@@ -79,57 +72,12 @@ __device__ __forceinline__ Box node_box(const float (*nb)[6], uint32_t i) {
   return {{nb[i][0], nb[i][1], nb[i][2]}, {nb[i][3], nb[i][4], nb[i][5]}};
 }
 
-__device__ __forceinline__ float half_area(const Box &b) {
-  const float x = b.hi[0] - b.lo[0], y = b.hi[1] - b.lo[1], z = b.hi[2] - b.lo[2];
-  return x * y + y * z + z * x;
-}
-
 __device__ __forceinline__ Box child_box(const float4 *ab, const BuildLds &S, uint16_t c) {
   return (c & kLeaf) ? leaf_box(ab, S.inst, c & ~kLeaf) : node_box(S.nb, c);
 }
 
 __device__ __forceinline__ float child_area(const BuildLds &S, uint16_t c) {
   return (c & kLeaf) ? -1.0f : half_area(node_box(S.nb, c));
-}
-
-// One 8-wide node: bounds and words of `cnt` children, empty slots after.
-// (All loops fully unrolled so the small arrays stay in registers.)
-__device__ void write_node(dnode *out, const Box *bx, const uint32_t *word, uint32_t cnt) {
-#pragma unroll
-  for (uint32_t c = 0; c < 8; c++) {
-    const bool e = c >= cnt;
-    out->minx[c] = e ? 3.402823466e38f : bx[c].lo[0];
-    out->maxx[c] = e ? -3.402823466e38f : bx[c].hi[0];
-    out->miny[c] = e ? 3.402823466e38f : bx[c].lo[1];
-    out->maxy[c] = e ? -3.402823466e38f : bx[c].hi[1];
-    out->minz[c] = e ? 3.402823466e38f : bx[c].lo[2];
-    out->maxz[c] = e ? -3.402823466e38f : bx[c].hi[2];
-    out->child[c] = e ? 0u : word[c];
-  }
-#pragma unroll
-  for (uint32_t o = 0; o < 8; o++) {
-    float dist[8];
-#pragma unroll
-    for (uint32_t c = 0; c < 8; c++) {
-      // near corner along the octant direction (bvh8.c ordering)
-      const float x = (o & 1) ? bx[c].lo[0] : bx[c].hi[0];
-      const float y = (o & 2) ? bx[c].lo[1] : bx[c].hi[1];
-      const float z = (o & 4) ? bx[c].lo[2] : bx[c].hi[2];
-      dist[c] = c >= cnt ? 3.402823466e38f
-                         : ((o & 1) ? x : -x) + ((o & 2) ? y : -y) + ((o & 4) ? z : -z);
-    }
-    // rank = position in the stable descending order (0 farthest, 7 nearest)
-    uint32_t inv = 0;
-#pragma unroll
-    for (uint32_t c = 0; c < 8; c++) {
-      uint32_t r = 0;
-#pragma unroll
-      for (uint32_t k = 0; k < 8; k++)
-        r += (dist[k] > dist[c] || (dist[k] == dist[c] && k < c)) ? 1u : 0u;
-      inv |= r << (3 * c);
-    }
-    out->inv[o] = inv;
-  }
 }
 
 __device__ __forceinline__ float block_min(BuildLds &S, int slot, float v) {
