@@ -62,6 +62,9 @@ struct CamArgs {
  * depends on the kernel variant, REND_LDS_STACK_MIN..RK_DEPTH0), deeper ones
  * spill to a global buffer of (REND_STACK_MAX - DEPTH) entries per thread. */
 #define REND_STACK_MAX      64   /* reference bound, rend.c:834 assert */
+/* Device arena limit: trees are addressed by 32-bit byte offsets from the
+ * arena's base (4 GiB less one 256-byte node, so every aligned offset fits) */
+#define RL_ARENA_MAX        0xffffff00ull
 #define REND_LDS_STACK_MIN  5
 #define RK_DEPTH0           15   /* variant 0 (4 waves per SIMD) */
 #define RK_TRACE_VARIANTS   4

@@ -449,7 +449,14 @@ static int arena_reserve(struct rl_scene *s, size_t bytes, size_t keep)
 {
 	if (bytes <= s->arena_cap)
 		return 0;
+	/* the traversal kernel keeps a tree's position in the arena as a 32-bit
+	 * byte offset */
+	if (bytes > RL_ARENA_MAX)
+		return rl_fail(REND_EINVAL, "scene needs a %zu-byte device arena; the limit is %llu",
+		  bytes, (unsigned long long)RL_ARENA_MAX);
 	size_t cap = bytes + bytes / 8 + 4096;
+	if (cap > RL_ARENA_MAX)
+		cap = RL_ARENA_MAX;
 	unsigned char *na = NULL;
 	hipError_t e = hipMalloc((void **)&na, cap);
 	if (e != hipSuccess)

@@ -51,6 +51,8 @@ struct RayS {
   bool fin;
 };
 
+typedef __attribute__((address_space(1))) uint32_t gu32;  // global memory
+
 __device__ __forceinline__ float xmax(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float xmin(float a, float b) { return a < b ? a : b; }
 
@@ -113,7 +115,7 @@ __device__ __noinline__ void report_error(uint32_t *diag, uint32_t code, const r
 // the hit mask indexed by rank.  FAST: every slab distance is known to be
 // finite (no NaN), so IEEE max/min (v_max3/v_min3) make the same decisions
 // as the reference's x86 MAXPS/MINPS operand rules.
-template <bool FAST>
+template <bool FAST, bool HOIST>
 __device__ __forceinline__ uint32_t slab8(const dnode *ndp, const RayS &cr, float t,
                                           float tmn[8], uint32_t ch[8], uint32_t &invw) {
   // nodes sit at 256-byte aligned arena offsets
@@ -161,6 +163,14 @@ __device__ __forceinline__ uint32_t slab8(const dnode *ndp, const RayS &cr, floa
     const uint32_t rank = (invw >> (3 * c)) & 7u;
     hmr |= (lo <= hi ? 1u : 0u) << rank;
   }
+  // The child words are only read when a child is hit; left alone, the
+  // compiler sinks their load into that branch, after the slab test, and a
+  // visit waits for a second memory round trip.  HOIST keeps the load with
+  // the bounds (-2 to -5 % per launch) where the register budget allows it:
+  // with the host-table reciprocal the kernel then spills to scratch.
+  if (HOIST)
+    asm volatile("" ::"v"(ch[0]), "v"(ch[1]), "v"(ch[2]), "v"(ch[3]), "v"(ch[4]), "v"(ch[5]),
+                 "v"(ch[6]), "v"(ch[7]));
   return hmr;
 }
 
@@ -208,10 +218,14 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   //   NODE        ofs is an interior node of the current tree
   //   LEAF        ofs is a BLAS leaf
   //   SETUP_INST  ofs is a TLAS leaf: transform the world ray, enter the BLAS
-  //   SETUP_WORLD the world ray's slab constants are needed (a new ray at the
-  //               TLAS root, or a TLAS node popped after a BLAS)
+  //   SETUP_WORLD the world ray's slab constants are needed (a TLAS node
+  //               popped after a BLAS)
+  //   SETUP_NEW   a new ray at the TLAS root: the refill loaded its record
+  //               into the world-ray registers and its tmax into tu (both
+  //               unread by the node step, which so never waits for that
+  //               load); the next setup step sets t and the slab constants
   enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_SETUP_INST = 3,
-                    ST_SETUP_WORLD = 4 };
+                    ST_SETUP_WORLD = 4, ST_SETUP_NEW = 5 };
   // leaf and setup steps run for at least this many waiting lanes, or once
   // fewer lanes than this are left at interior nodes
   const uint32_t kStepBatch = (flags >> 8) & 0xffu ? (flags >> 8) & 0xffu : RK_STEP_BATCH;
@@ -226,7 +240,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   float t = 0.0f, tu = 0.0f, tv = 0.0f;
   uint32_t hid = 0xffffffffu;
   uint32_t ofs = 0, sp = 0, blas_sp = 0, instid = 0, iter = 0;
-  uint64_t base = tlas;
+  uint32_t base = (uint32_t)tlas;  // arena offset of the current tree (< 4 GiB)
   bool in_blas = false;
   bool occluded = false;
   uint64_t c_nodes = 0, c_leaves = 0, c_insts = 0, c_rays = 0;
@@ -254,13 +268,13 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         report_error(diag, RK_ERR_ITER, rays, rid, iter);
         done = true;
       } else {
-        const dnode *nd = (const dnode *)(arena + base + ofs);
+        const dnode *nd = (const dnode *)(arena + (base + ofs));
         float tmn[8];
         uint32_t ch[8], invw;
         // the path is chosen before the node is read, so each is one block
         const uint32_t hmr = __ballot(!(cr.fin && t == t) && state == ST_NODE) == 0
-                                 ? slab8<true>(nd, cr, t, tmn, ch, invw)
-                                 : slab8<false>(nd, cr, t, tmn, ch, invw);
+                                 ? slab8<true, RCP_IEEE>(nd, cr, t, tmn, ch, invw)
+                                 : slab8<false, RCP_IEEE>(nd, cr, t, tmn, ch, invw);
         if (COUNT) c_nodes++;
 
         const uint32_t hitcnt = __builtin_popcount(hmr);
@@ -328,7 +342,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           report_error(diag, RK_ERR_ITER, rays, rid, iter);
           done = true;
         } else {
-          const dleaf *lf = (const dleaf *)(arena + base + (ofs & ~DNODE_LEAF));
+          const dleaf *lf = (const dleaf *)(arena + (base + (ofs & ~DNODE_LEAF)));
           float4 v0x = *(const float4 *)lf->v0x, v0y = *(const float4 *)lf->v0y,
                  v0z = *(const float4 *)lf->v0z;
           float4 e0x = *(const float4 *)lf->e0x, e0y = *(const float4 *)lf->e0y,
@@ -396,12 +410,14 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
 
     // ---- setup step: instance entry (transform, then the BLAS ray's slab
     // constants) and world-ray constants (new rays, TLAS re-entry) ----------
-    const uint64_t m_setup =
-        __ballot((state == ST_SETUP_INST || state == ST_SETUP_WORLD) && !need_pop && !done);
+    const bool in_setup = state == ST_SETUP_INST || state == ST_SETUP_WORLD ||
+                          state == ST_SETUP_NEW;
+    const uint64_t m_setup = __ballot(in_setup && !need_pop && !done);
     if (m_setup && ((uint32_t)__popcll(m_setup) >= kStepBatch || n_node < kStepBatch)) {
       if (COUNT) c_setupx++;
-      if ((state == ST_SETUP_INST || state == ST_SETUP_WORLD) && !need_pop && !done) {
-        if (state == ST_SETUP_WORLD) {
+      if (in_setup && !need_pop && !done) {
+        if (state != ST_SETUP_INST) {
+          if (state == ST_SETUP_NEW) { t = tu; tu = 0.0f; }
           ray_setup(cr, wo[0], wo[1], wo[2], wd[0], wd[1], wd[2]);
           state = ST_NODE;
         } else if (++iter > max_iter) {
@@ -415,6 +431,11 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           const float4 m1 = *(const float4 *)&in->inv[4];
           const float4 m2 = *(const float4 *)&in->inv[8];
           const uint64_t blas = in->blas;
+          // wait for the BLAS offset here, beside the matrix rows: left
+          // outstanding, the next node step (which needs it for its address)
+          // would have to drain the whole vector-memory counter, hit stores
+          // and refill loads included
+          asm volatile("" ::"v"(blas));
           if (COUNT) c_insts++;
           const float ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
           const float oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
@@ -428,7 +449,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
             ray_setup(cr, ox, oy, oz, dx, dy, dz);
           in_blas = true;
           blas_sp = sp;
-          base = blas;
+          base = (uint32_t)blas;
           ofs = 0;
           state = ST_NODE;
         }
@@ -443,7 +464,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           // back in the TLAS: the world ray's constants are needed again
           // before the next TLAS node (not for an instance entry or the end)
           in_blas = false;
-          base = tlas;
+          base = (uint32_t)tlas;
           world = true;
         }
         if (sp == 0) { done = true; break; }
@@ -451,8 +472,11 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         uint2 e = s_stack[sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH][tid];
         if (sp >= (uint32_t)DEPTH) {
           // volatile keeps the compiler from merging this load with the LDS
-          // one into a single generic (flat) load
-          const volatile uint32_t *q = (const volatile uint32_t *)&myspill[(sp - DEPTH) * sstride];
+          // one; the global address space keeps it a global (not flat)
+          // load: a flat load counts against both the vector-memory and
+          // the LDS counters, and while one may be outstanding every later
+          // wait in the loop has to drain both counters completely
+          const volatile gu32 *q = (const volatile gu32 *)&myspill[(sp - DEPTH) * sstride];
           e.x = q[0];
           e.y = q[1];
         }
@@ -508,14 +532,14 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           const float4 b = *(const float4 *)&rays[r].dx;
           wo[0] = a.x; wo[1] = a.y; wo[2] = a.z;
           wd[0] = b.x; wd[1] = b.y; wd[2] = b.z;
-          t = a.w;
-          tu = tv = 0.0f;
+          tu = a.w;  // tmax, moved to t by the SETUP_NEW step
+          tv = 0.0f;
           hid = 0xffffffffu;
           ofs = 0; sp = 0; blas_sp = 0; iter = 0;
-          base = tlas;
+          base = (uint32_t)tlas;
           in_blas = false;
           occluded = false;
-          state = ST_SETUP_WORLD;
+          state = ST_SETUP_NEW;
           if (COUNT) c_rays++;
         }
       }
