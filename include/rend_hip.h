@@ -293,10 +293,13 @@ int          rend_hip_gen_camrays(struct rdata *rd, struct rray *d_rays,
 
 /* The reference's own primary rays for a frame that starts after `samples`
  * accumulated samples (make_camray_pckt in rend_render, rend.c:3603-3660,
- * 3293-3402, pinhole cameras): blocks of the rend_staticinit size
- * (rd->width / bszx by rd->height / bszy of them, edge remainders not
+ * 3293-3402, pinhole and thin-lens cameras): blocks of the rend_staticinit
+ * size (rd->width / bszx by rd->height / bszy of them, edge remainders not
  * rendered, as in the reference), 4x2-pixel packets in Morton order, the
- * reference's xorshift128+ jitter stream and the host RSQRTPS, bit for bit.
+ * reference's xorshift128+ jitter stream, the host RSQRTPS and, for a thin
+ * lens, this host's libmvec cosf/sinf of the lens angle (tabulated over its
+ * 2^23 values on first use: 64 MiB of device memory kept until the process
+ * ends), bit for bit.
  * Ray i = (block * packets_per_block + packet) * 8 + lane; tmax = FLT_MAX.
  * d_rays holds rend_hip_camrays_ref_count(rd) records. */
 unsigned long long rend_hip_camrays_ref_count(struct rdata *rd);

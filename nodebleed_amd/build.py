@@ -69,7 +69,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs),
-          "-o", str(tmp), "-lpthread", "-lm"])
+          "-o", str(tmp), "-lpthread", "-lmvec", "-lm"])
     os.replace(tmp, OUT)
     if verbose:
         print(OUT)

@@ -13,6 +13,8 @@
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
 
+#include <immintrin.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -25,7 +27,7 @@
 
 int rk_camrays_ref_setup(const unsigned *tab, void *stream);
 int rk_camrays_ref(const struct CamArgs *c, unsigned bszx, unsigned bszy, unsigned blkcntx,
-                   unsigned nblk, unsigned samples, void *out, void *stream);
+                   unsigned nblk, unsigned samples, const void *lens, void *out, void *stream);
 int rk_camrays(const struct CamArgs *c, unsigned rw, unsigned rh,
                unsigned spp, unsigned seed, unsigned frame, void *out,
                void *stream);
@@ -145,6 +147,7 @@ static int frame_buffers(struct rl_scene *s, size_t nrays)
 }
 
 static int rsqrt_table_ready(hipStream_t st);
+static int lens_table_ready(const struct rcam *c, const void **lens);
 
 static int is_pow2(unsigned v) { return v && !(v & (v - 1)); }
 
@@ -152,10 +155,11 @@ static int is_pow2(unsigned v) { return v && !(v & (v - 1)); }
  * Like the reference it traces one primary ray per pixel of the block grid
  * per frame and divides the accumulator by spp + samples.  With a pinhole
  * camera and blocks made of 4x2 packets in a Morton square (or 2:1) the rays
- * are the reference's own (make_camray_pckt with its xorshift128+ streams
- * and RSQRTPS), so the frame equals the reference's single-ray shading of
- * its primaries bit for bit; otherwise (thin lens, other block shapes) the
- * rays come from k_camrays' jittered, exactly normalised camera. */
+ * are the reference's own (make_camray_pckt with its xorshift128+ streams,
+ * RSQRTPS and, for a thin lens, libmvec's cos/sin), so the frame equals the
+ * reference's single-ray shading of its primaries bit for bit; with other
+ * block shapes the rays come from k_camrays' jittered, exactly normalised
+ * camera. */
 static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
                           unsigned *rays_out)
 {
@@ -165,7 +169,7 @@ static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
 	if (r)
 		return r;
 	const unsigned bx = rl_blkszx(), by = rl_blkszy();
-	const int refrays = !(rd->cam.tanfangle > 0.0f) && bx % 4 == 0 && by % 2 == 0 &&
+	const int refrays = bx % 4 == 0 && by % 2 == 0 &&
 	  is_pow2(by / 2) && (bx / 4 == by / 2 || bx / 4 == by);
 	size_t nrays = (size_t)rw * rh;
 	if ((r = frame_buffers(s, nrays)))
@@ -175,9 +179,10 @@ static int render_normals(struct rl_scene *s, unsigned rw, unsigned rh,
 	struct rl_frame *f = s->frame;
 	const unsigned blkcntx = rd->width / bx, nblk = blkcntx * (rd->height / by);
 	if (refrays) {
-		if ((r = rsqrt_table_ready(st)))
+		const void *lens = NULL;
+		if ((r = rsqrt_table_ready(st)) || (r = lens_table_ready(&rd->cam, &lens)))
 			return r;
-		if ((r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, rd->samples, f->rays, st)))
+		if ((r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, rd->samples, lens, f->rays, st)))
 			return rl_hipfail(r, "reference camera rays");
 	} else if ((r = rk_camrays(&ca, rw, rh, 1, 0x2545f491u, rd->samples, f->rays, st))) {
 		return rl_hipfail(r, "camera rays");
@@ -320,6 +325,70 @@ static int rsqrt_table_ready(hipStream_t st)
 	return 0;
 }
 
+/* glibc libmvec, AVX2 variants: the cos/sin the reference's thin-lens
+ * branch calls (rend.c:3328-3329) */
+__m256 _ZGVdN8v_cosf(__m256 x);
+__m256 _ZGVdN8v_sinf(__m256 x);
+
+#define RL_LENS_N (1u << 23)  /* values of the 23-bit lens-angle mantissa */
+
+static void *g_lens;          /* device table of RL_LENS_N (cos, sin) pairs */
+static int g_lens_dev = -1;   /* device it lives on */
+
+/* The reference's lens angle is 2 pi u3 with u3 = m 2^-23, m the 23 random
+ * mantissa bits of randf8, so cos and sin take only 2^23 values: this host's
+ * libmvec results for every m, computed once (8 lanes per call, as the
+ * reference calls them) and kept on the device (64 MiB) until the process
+ * ends.  Only needed for a thin-lens camera; *lens stays NULL otherwise. */
+static int lens_table_ready(const struct rcam *c, const void **lens)
+{
+	*lens = NULL;
+	if (!(c->tanfangle > 0.0f))
+		return 0;
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e)
+		return rl_hipfail(e, "device");
+	if (g_lens_dev != dev) {
+		float *h = malloc((size_t)RL_LENS_N * 2 * sizeof(float));
+		if (!h)
+			return rl_fail(REND_ENOMEM, "lens table (host)");
+		const __m256 twopi = _mm256_set1_ps(6.28318531f);
+		const __m256i one = _mm256_set1_epi32(127 << 23);
+		for (uint32_t m = 0; m < RL_LENS_N; m += 8) {
+			const __m256i k = _mm256_add_epi32(_mm256_set1_epi32((int)m),
+			  _mm256_setr_epi32(0, 1, 2, 3, 4, 5, 6, 7));
+			/* randf8: (m | 1.0f's exponent) - 1.0f */
+			const __m256 u = _mm256_sub_ps(_mm256_castsi256_ps(_mm256_or_si256(k, one)),
+			  _mm256_castsi256_ps(one));
+			const __m256 th = _mm256_mul_ps(twopi, u);
+			float cv[8], sv[8];
+			_mm256_storeu_ps(cv, _ZGVdN8v_cosf(th));
+			_mm256_storeu_ps(sv, _ZGVdN8v_sinf(th));
+			for (int l = 0; l < 8; l++) {
+				h[2 * (m + l)] = cv[l];
+				h[2 * (m + l) + 1] = sv[l];
+			}
+		}
+		void *d = NULL;
+		e = hipMalloc(&d, (size_t)RL_LENS_N * 2 * sizeof(float));
+		if (!e)
+			e = hipMemcpy(d, h, (size_t)RL_LENS_N * 2 * sizeof(float), hipMemcpyHostToDevice);
+		free(h);
+		if (e) {
+			if (d)
+				hipFree(d);
+			return rl_hipfail(e, "lens table");
+		}
+		if (g_lens)
+			hipFree(g_lens);  /* a table of a device no longer current */
+		g_lens = d;
+		g_lens_dev = dev;
+	}
+	*lens = g_lens;
+	return 0;
+}
+
 unsigned long long rend_hip_camrays_ref_count(struct rdata *rd)
 {
 	const unsigned bx = rl_blkszx(), by = rl_blkszy();
@@ -337,15 +406,14 @@ int rend_hip_gen_camrays_ref(struct rdata *rd, struct rray *d_rays, unsigned int
 	const unsigned bx = rl_blkszx(), by = rl_blkszy();
 	if (bx % 4 || by % 2)
 		return rl_fail(REND_EINVAL, "block %ux%u is not made of 4x2 packets", bx, by);
-	if (rd->cam.tanfangle > 0.0f)
-		return rl_fail(REND_EINVAL, "reference-exact rays need a pinhole camera");
 	hipStream_t st = stream ? (hipStream_t)stream : G.stream;
-	if ((r = rsqrt_table_ready(st)))
+	const void *lens = NULL;
+	if ((r = rsqrt_table_ready(st)) || (r = lens_table_ready(&rd->cam, &lens)))
 		return r;
 	struct CamArgs ca;
 	rl_camargs(&rd->cam, &ca);
 	const unsigned blkcntx = rd->width / bx, nblk = blkcntx * (rd->height / by);
-	r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, samples, d_rays, st);
+	r = rk_camrays_ref(&ca, bx, by, blkcntx, nblk, samples, lens, d_rays, st);
 	return r ? rl_hipfail(r, "reference camera rays") : 0;
 }
 

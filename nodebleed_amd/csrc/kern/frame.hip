@@ -6,9 +6,10 @@
 //               uses the approximate RSQRTPS) and jitter from a counter-based
 //               hash instead of xorshift128+ packets, so images agree with the
 //               reference statistically, not bit for bit.
-//   k_camrays_ref  the reference's own pinhole primary rays, bit for bit
-//               (xorshift128+ packets, host RSQRTPS table): what rend_render
-//               traces for a pinhole camera.
+//   k_camrays_ref  the reference's own primary rays, bit for bit
+//               (xorshift128+ packets, host RSQRTPS table, for a thin lens
+//               the host libmvec's cos/sin of the lens angle from a table):
+//               what rend_render traces.
 //   k_shade_normals, k_shade_normals_ref  trace_rays/calc_nrm
 //               (rend.c:3404-3440, 3109-3116): colour (n * 0.5 + 0.5) *
 //               albedo, background on a miss, for pixel-major rays or for
@@ -322,22 +323,28 @@ __device__ __forceinline__ uint32_t mort_dec(uint32_t v) {
 
 __global__ void k_camrays_ref(CamArgs c, uint32_t bszx, uint32_t bszy, uint32_t blkcntx,
                               uint32_t nblk, uint32_t npk, uint32_t samples,
-                              rray *__restrict__ out) {
+                              const float2 *__restrict__ lens, rray *__restrict__ out) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nblk * npk) return;
   const uint32_t blk = t / npk, k = t % npk;
+  // draws per packet: u0, u1 (pixel jitter), and u2, u3 (lens) for a thin lens
+  const int nd = c.dof ? 4 : 2;
   Xs128 g[4];
   g[0].s0 = (uint64_t)(int64_t)(((int)blk + 13) * 1571);
   g[0].s1 = (uint64_t)(samples * 23u);
   for (int l = 1; l < 4; l++) g[l] = xs128_jump(g[l - 1]);
-  for (uint32_t i = 0; i < 2u * k; i++)
+  for (uint32_t i = 0; i < (uint32_t)nd * k; i++)
     for (int l = 0; l < 4; l++) xs128_next(g[l]);
-  float u[2][8];
-  for (int d = 0; d < 2; d++)
+  float u[4][8];
+  // mantissa bits of u3, the index of the lens angle 2 pi u3 in the table
+  uint32_t m3[8];
+  for (int d = 0; d < nd; d++)
     for (int l = 0; l < 4; l++) {
       const uint64_t v = xs128_next(g[l]);
-      u[d][2 * l] = __uint_as_float(((uint32_t)v >> 9) | 0x3f800000u) - 1.0f;
-      u[d][2 * l + 1] = __uint_as_float(((uint32_t)(v >> 32) >> 9) | 0x3f800000u) - 1.0f;
+      const uint32_t lo = (uint32_t)v >> 9, hi = (uint32_t)(v >> 32) >> 9;
+      u[d][2 * l] = __uint_as_float(lo | 0x3f800000u) - 1.0f;
+      u[d][2 * l + 1] = __uint_as_float(hi | 0x3f800000u) - 1.0f;
+      if (d == 3) { m3[2 * l] = lo; m3[2 * l + 1] = hi; }
     }
   const uint32_t x = (blk % blkcntx) * bszx + mort_dec(k) * 4u;
   const uint32_t y = (blk / blkcntx) * bszy + mort_dec(k >> 1) * 2u;
@@ -355,12 +362,28 @@ __global__ void k_camrays_ref(CamArgs c, uint32_t bszx, uint32_t bszy, uint32_t 
     const float rx = __builtin_fmaf(c.ri[0], sx, -ux);
     const float ry = __builtin_fmaf(c.ri[1], sx, -uy);
     const float rz = __builtin_fmaf(c.ri[2], sx, -uz);
-    const float tx = __builtin_fmaf(c.fwd[0], c.fdist, rx);
-    const float ty = __builtin_fmaf(c.fwd[1], c.fdist, ry);
-    const float tz = __builtin_fmaf(c.fwd[2], c.fdist, rz);
+    float ox = c.eye[0], oy = c.eye[1], oz = c.eye[2], tx, ty, tz;
+    if (c.dof) {
+      // lens sample (rend.c:3320-3337): radius sqrt(u2), angle 2 pi u3 with
+      // cos and sin from the host's libmvec, tabulated over all 2^23 u3
+      const float rad = sqrtf(u[2][l]);
+      const float2 cs = lens[m3[l]];
+      const float lx = cs.x * rad, ly = cs.y * rad;
+      ox = __builtin_fmaf(__builtin_fmaf(c.ri[0], lx, c.up[0] * ly), c.focrad, c.eye[0]);
+      oy = __builtin_fmaf(__builtin_fmaf(c.ri[1], lx, c.up[1] * ly), c.focrad, c.eye[1]);
+      oz = __builtin_fmaf(__builtin_fmaf(c.ri[2], lx, c.up[2] * ly), c.focrad, c.eye[2]);
+      // towards the pixel's point on the focal plane (rend.c:3339-3366)
+      tx = (__builtin_fmaf(c.fwd[0], c.fdist, c.eye[0]) + rx) - ox;
+      ty = (__builtin_fmaf(c.fwd[1], c.fdist, c.eye[1]) + ry) - oy;
+      tz = (__builtin_fmaf(c.fwd[2], c.fdist, c.eye[2]) + rz) - oz;
+    } else {
+      tx = __builtin_fmaf(c.fwd[0], c.fdist, rx);
+      ty = __builtin_fmaf(c.fwd[1], c.fdist, ry);
+      tz = __builtin_fmaf(c.fwd[2], c.fdist, rz);
+    }
     const float rl = rsqrt_host(__builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz)));
     float4 *o = (float4 *)&out[(size_t)t * 8u + (uint32_t)l];
-    o[0] = make_float4(c.eye[0], c.eye[1], c.eye[2], 3.402823466e38f);
+    o[0] = make_float4(ox, oy, oz, 3.402823466e38f);
     o[1] = make_float4(tx * rl, ty * rl, tz * rl, 0.0f);
   }
 }
@@ -396,12 +419,16 @@ extern "C" int rk_camrays_ref_setup(const unsigned *tab, void *stream) {
                                      hipMemcpyHostToDevice, (hipStream_t)stream);
 }
 
+// lens: 2^23 (cos, sin) pairs of the thin-lens angle, needed when c->dof.
 extern "C" int rk_camrays_ref(const CamArgs *c, unsigned bszx, unsigned bszy, unsigned blkcntx,
-                              unsigned nblk, unsigned samples, void *out, void *stream) {
+                              unsigned nblk, unsigned samples, const void *lens, void *out,
+                              void *stream) {
   const unsigned npk = bszx * bszy / 8u, n = nblk * npk;
   if (n == 0) return 0;
+  if (c->dof && !lens) return (int)hipErrorInvalidValue;
   hipLaunchKernelGGL(k_camrays_ref, dim3((n + 127) / 128), dim3(128), 0, (hipStream_t)stream,
-                     *c, bszx, bszy, blkcntx, nblk, npk, samples, (rray *)out);
+                     *c, bszx, bszy, blkcntx, nblk, npk, samples, (const float2 *)lens,
+                     (rray *)out);
   return (int)hipGetLastError();
 }
 

@@ -1,11 +1,16 @@
 /*
  * ORACLE (test infrastructure only).  C restatement of the reference's
- * primary-ray generation for pinhole cameras, used to check the GPU's
- * reference-exact camera rays.  Follows:
+ * primary-ray generation for pinhole and thin-lens cameras, used to check
+ * the GPU's reference-exact camera rays.  Follows:
  *   rend_render's block walk and per-block seed   rend.c:3603-3640
  *   4x2-pixel packets in Morton order              rend.c:131-162, util.c:133-145
  *   rand8_init / xorshift128+ / jump / randf8      util.c:29-75, 99-107
  *   make_camray_pckt, pinhole branch               rend.c:3293-3317, 3377-3401
+ *   make_camray_pckt, thin-lens branch             rend.c:3320-3376: lens
+ *                                                  angle 2pi * u3 through
+ *                                                  glibc libmvec's 8-wide
+ *                                                  cosf/sinf, the functions
+ *                                                  the reference calls
  *   _mm256_rsqrt_ps                                this host's RSQRTPS (mode 0)
  *                                                  or the build host's table
  *                                                  (mode 2, gen_rsqrt_table.c rule)
@@ -23,6 +28,10 @@
 #include "rend.h"
 #include "rend_rays.h"
 #include "../nodebleed_amd/csrc/rsqrt_table.h"
+
+/* glibc libmvec, AVX2 variants (what rend.c:3328-3329 calls) */
+__m256 _ZGVdN8v_cosf(__m256 x);
+__m256 _ZGVdN8v_sinf(__m256 x);
 
 static const uint32_t rsq_tab[8192] = RSQRT_TABLE_INIT;
 static int g_rsq_mode;  /* 0 host instruction, 2 build host's table */
@@ -107,6 +116,8 @@ void or_camrays(const struct rdata *rd, unsigned bszx, unsigned bszy, unsigned s
 	const float fw[3] = {lane0(c->fwdx8), lane0(c->fwdy8), lane0(c->fwdz8)};
 	const float rw = lane0(c->rw8), rh = lane0(c->rh8), asp = lane0(c->aspect8);
 	const float fd = lane0(c->fdist8), ff = lane0(c->fovfdist8);
+	const float fr = lane0(c->focrad8);
+	const int lens = c->tanfangle > 0.0f;
 	const unsigned blkcntx = rd->width / bszx, npk = bszx * bszy / 8;
 	struct xs g[4];
 	g[0].s0 = (uint64_t)(int64_t)(((int)blk + 13) * 1571);
@@ -114,13 +125,21 @@ void or_camrays(const struct rdata *rd, unsigned bszx, unsigned bszy, unsigned s
 	for (int l = 1; l < 4; l++)
 		g[l] = xs_jump(g[l - 1]);
 	for (unsigned k = 0; k < npk; k++) {
-		float u[2][8];
-		for (int d = 0; d < 2; d++)
+		/* u0, u1: pixel jitter; u2, u3: lens sample (thin lens only) */
+		float u[4][8];
+		for (int d = 0; d < (lens ? 4 : 2); d++)
 			for (int l = 0; l < 4; l++) {
 				const uint64_t v = xs_next(&g[l]);
 				u[d][2 * l] = f32(((uint32_t)v >> 9) | 0x3f800000u) - 1.0f;
 				u[d][2 * l + 1] = f32(((uint32_t)(v >> 32) >> 9) | 0x3f800000u) - 1.0f;
 			}
+		float lc[8], ls[8];
+		if (lens) {
+			const __m256 th = _mm256_mul_ps(_mm256_set1_ps(6.28318531f),
+			  _mm256_loadu_ps(u[3]));
+			_mm256_storeu_ps(lc, _ZGVdN8v_cosf(th));
+			_mm256_storeu_ps(ls, _ZGVdN8v_sinf(th));
+		}
 		const unsigned x = (blk % blkcntx) * bszx + mort(k) * 4;
 		const unsigned y = (blk / blkcntx) * bszy + mort(k >> 1) * 2;
 		for (int l = 0; l < 8; l++) {
@@ -132,14 +151,23 @@ void or_camrays(const struct rdata *rd, unsigned bszx, unsigned bszy, unsigned s
 			sx = sx * ff;
 			sy = sy * ff;
 			sx = sx * asp;
-			float t[3];
-			for (int a = 0; a < 3; a++)
-				t[a] = fmaf(fw[a], fd, fmaf(ri[a], sx, -(up[a] * sy)));
+			float t[3], o[3] = {eye[0], eye[1], eye[2]};
+			if (lens) {
+				const float rad = sqrtf(u[2][l]);
+				const float rx = lc[l] * rad, ry = ls[l] * rad;
+				for (int a = 0; a < 3; a++) {
+					o[a] = fmaf(fmaf(ri[a], rx, up[a] * ry), fr, eye[a]);
+					t[a] = (fmaf(fw[a], fd, eye[a]) + fmaf(ri[a], sx, -(up[a] * sy))) - o[a];
+				}
+			} else {
+				for (int a = 0; a < 3; a++)
+					t[a] = fmaf(fw[a], fd, fmaf(ri[a], sx, -(up[a] * sy)));
+			}
 			const float rl = cam_rsqrt(fmaf(t[0], t[0], fmaf(t[1], t[1], t[2] * t[2])));
 			struct rray *r = &out[k * 8 + (unsigned)l];
-			r->ox = eye[0];
-			r->oy = eye[1];
-			r->oz = eye[2];
+			r->ox = o[0];
+			r->oy = o[1];
+			r->oz = o[2];
 			r->tmax = 3.402823466e38f;
 			r->dx = t[0] * rl;
 			r->dy = t[1] * rl;

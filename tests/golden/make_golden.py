@@ -8,13 +8,13 @@ records
     reference's rend_prepstatic/rend_prepdynamic write, and
   * rays with the reference's intersect_tlas hits and intersect_any_tlas
     results (random, aimed and edge-case rays),
-into tests/golden/ref_<name>.npz, and for two scenes three whole frames of
-the reference's rend_render (normal-tint shading, 64x32 blocks, its own
-camera rays) and of the same block walk traced with its single-ray
-trace_rays, into tests/golden/frames_<name>.npz.  The fixtures are data
-only.
+into tests/golden/ref_<name>.npz, and for three scenes (one through a thin
+lens) three whole frames of the reference's rend_render (normal-tint
+shading, 64x32 blocks, its own camera rays) and of the same block walk
+traced with its single-ray trace_rays, into tests/golden/frames_<name>.npz.
+The fixtures are data only.
 
-    python tests/golden/make_golden.py [frames]
+    python tests/golden/make_golden.py [frames [name ...]]
 """
 from __future__ import annotations
 
@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 from nodebleed_amd import Scene  # noqa: E402
 from oracle_lib import RefScene, reference  # noqa: E402
-from raygen import aimed_rays, edge_rays, random_rays  # noqa: E402
+from raygen import aimed_rays, edge_rays, random_rays, set_lens_camera  # noqa: E402
 
 #: name -> (config, detail); small enough to ship and to check in seconds
 SCENES = {"terrain": (2, 0.05), "box": (1, 0.25), "instances": (3, 0.03)}
@@ -49,15 +49,21 @@ def regions(scene: Scene):
     return out
 
 
-#: name -> (config, detail, width, height) of the frame fixtures
-FRAME_SCENES = {"box": (1, 1.0, 128, 128), "terrain": (2, 0.2, 192, 128)}
+#: name -> (config, detail, width, height, thin lens (focus angle, focus
+#: distance) or None) of the frame fixtures
+FRAME_SCENES = {"box": (1, 1.0, 128, 128, None), "terrain": (2, 0.2, 192, 128, None),
+                "box_lens": (1, 1.0, 128, 128, (6.0, 3.5))}
 FRAMES = 3
 BLOCK = (64, 32)
 
 
-def frames():
-    for name, (cfg, det, w, h) in FRAME_SCENES.items():
+def frames(names=None):
+    for name, (cfg, det, w, h, lens) in FRAME_SCENES.items():
+        if names and name not in names:
+            continue
         s = Scene(config=cfg, detail=det, width=w, height=h)
+        if lens:
+            set_lens_camera(s, *lens)
         r = RefScene(s)
         buf_p, acc_p, rays_p = r.frames(s, FRAMES, block=BLOCK, threads=4)
         buf_s, acc_s, rays_s = r.frames(s, FRAMES, block=BLOCK, single=True)
@@ -66,6 +72,7 @@ def frames():
         diff = np.flatnonzero((acc_p != acc_s).any(axis=2) | (buf_p != buf_s))
         np.savez_compressed(HERE / f"frames_{name}.npz", config=cfg, detail=det,
                             width=w, height=h, frames=FRAMES, block=np.array(BLOCK),
+                            lens=np.array(lens if lens else (0.0, 0.0), np.float32),
                             buf_single=buf_s, acc_single=acc_s, rays_single=rays_s,
                             rays_packets=rays_p, packet_diff_px=diff,
                             buf_packets_diff=buf_p.reshape(-1)[diff],
@@ -79,8 +86,8 @@ def frames():
 def main():
     if reference() is None:
         subprocess.run(["make", "-C", str(ROOT / "oracle"), "ref"], check=True)
-    if sys.argv[1:] == ["frames"]:
-        frames()
+    if sys.argv[1:2] == ["frames"]:
+        frames(sys.argv[2:])
         return
     frames()
     for name, (cfg, det) in SCENES.items():

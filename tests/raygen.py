@@ -112,3 +112,17 @@ def ulp_diff(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
     ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
     return np.abs(ia - ib)
+
+
+def set_lens_camera(scene, focangle: float, focdist: float, vfov: float = 45.0) -> None:
+    """Gives the scene's rdata a thin-lens camera (focus angle and distance,
+    as main.c's upd_rcam reads them) at the current eye and view direction."""
+    import ctypes as C
+    c = scene.camera()
+    eye = np.ascontiguousarray(c[0:3], np.float32)
+    target = np.ascontiguousarray(c[0:3] + c[9:12], np.float32)
+    v = scene.view()
+    rc = scene.L.rs_set_rcam(scene.rd, eye.ctypes.data_as(C.POINTER(C.c_float)),
+                             target.ctypes.data_as(C.POINTER(C.c_float)), vfov, focdist,
+                             focangle, v.width, v.height)
+    assert rc == 0

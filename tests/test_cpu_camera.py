@@ -10,6 +10,7 @@ import pytest
 
 from nodebleed_amd import Scene, lib
 from oracle_lib import oracle, or_camrays, ref_camrays, reference
+from raygen import set_lens_camera
 
 
 def f32(u):
@@ -79,6 +80,35 @@ def test_camrays_match_reference_bit_exact(terrain, bs, samples):
     want = ref_camrays(terrain, bx, by, samples, blocks)
     got = or_camrays(terrain, bx, by, samples, blocks)
     assert want.tobytes() == got.tobytes()
+
+
+@pytest.mark.parametrize("bs", [(32, 8), (64, 32)])
+@pytest.mark.parametrize("samples", [0, 37])
+def test_thin_lens_camrays_match_reference_bit_exact(bs, samples):
+    """The thin-lens branch (rend.c:3320-3376): lens sample from two more
+    xorshift128+ draws per packet, the angle through glibc libmvec's cosf and
+    sinf, origin on the lens, direction towards the focal plane."""
+    if reference() is None:
+        pytest.skip("reference harness not built")
+    s = Scene(config=2, detail=0.05, width=256, height=128)
+    try:
+        set_lens_camera(s, focangle=4.0, focdist=2.5)
+        assert s.camera()[17] > 0  # focrad: the lens branch runs
+        bx, by = bs
+        nblk = (256 // bx) * (128 // by)
+        blocks = [0, 3, nblk // 2, nblk - 1]
+        want = ref_camrays(s, bx, by, samples, blocks)
+        got = or_camrays(s, bx, by, samples, blocks)
+        assert want.tobytes() == got.tobytes()
+        # rays leave from a disk of radius focrad around the eye and meet the
+        # pinhole ray of the same jittered pixel on the focal plane
+        o = np.stack([got["ox"], got["oy"], got["oz"]], -1).reshape(-1, 3).astype(np.float64)
+        c = s.camera().astype(np.float64)
+        r = np.linalg.norm(o - c[0:3], axis=1)
+        assert r.max() <= c[17] * (1 + 1e-5) and r.max() > 0.5 * c[17]
+        assert np.abs((o - c[0:3]) @ c[9:12]).max() < 1e-5  # the lens is in the image plane
+    finally:
+        s.close()
 
 
 def test_camray_properties(terrain):

@@ -7,7 +7,7 @@ ARGB, packets and single rays agreeing on all but a few pixels)."""
 import numpy as np
 import pytest
 
-from frames_ref import NAMES, load
+from frames_ref import NAMES, load, scene_of
 from nodebleed_amd import Scene
 from oracle_lib import RefScene, reference
 
@@ -29,8 +29,7 @@ def test_frame_fixture_invariants(name):
 @pytest.mark.parametrize("name", NAMES)
 def test_frame_fixture_matches_live_reference(name):
     f = load(name)
-    s = Scene(config=int(f["config"]), detail=float(f["detail"]), width=int(f["width"]),
-              height=int(f["height"]))
+    s = scene_of(f)
     r = RefScene(s)
     try:
         blk = tuple(int(b) for b in f["block"])

@@ -9,7 +9,7 @@ import pytest
 
 from nodebleed_amd import REND_RCP_HOST_TABLE, REND_RCP_XEON_TABLE, RAY, Scene, check, lib
 from oracle_lib import or_camrays, or_trace, ref_camrays, reference
-from raygen import mismatches
+from raygen import mismatches, set_lens_camera
 
 pytestmark = pytest.mark.gpu
 
@@ -40,6 +40,49 @@ def test_reference_camrays_bit_exact(bs, samples):
         # the primary pass: these rays through the GPU traversal, bit-exact
         # with the oracle on a sample
         rng = np.random.default_rng(samples)
+        idx = np.sort(rng.choice(n, 30000, replace=False))
+        say("trace a sample of them")
+        hits = s.trace(got[idx])
+        h_want, _ = or_trace(s, got[idx])
+        assert mismatches(hits, h_want) == {"id": 0, "t": 0, "u": 0, "v": 0}
+    finally:
+        L.rend_hip_free(d)
+        L.rend_staticinit(64, 32)
+        s.close()
+
+
+@pytest.mark.parametrize("bs,samples", [((64, 32), 0), ((32, 8), 9)])
+def test_reference_thin_lens_camrays_bit_exact(bs, samples):
+    """Thin-lens camera (rend.c:3320-3376): two more draws per packet, the
+    lens angle's cos/sin from this host's libmvec through the table, origin
+    on the lens, direction towards the focal plane; rays equal the oracle's
+    (which calls libmvec itself), and on this host the reference's own."""
+    L = lib()
+    bx, by = bs
+    s = Scene(config=2, detail=0.25, width=1920, height=1080)
+    set_lens_camera(s, focangle=4.0, focdist=2.5)
+    L.rend_staticinit(bx, by)
+    n = L.rend_hip_camrays_ref_count(s.rd)
+    nblk = (1920 // bx) * (1080 // by)
+    d = L.rend_hip_malloc(n * RAY.itemsize)
+    try:
+        say(f"thin-lens reference camera rays {bx}x{by} blocks, samples {samples}: {n} rays")
+        check(L.rend_hip_gen_camrays_ref(s.rd, d, samples, None))
+        check(L.rend_hip_sync())
+        got = np.zeros(n, RAY)
+        check(L.rend_hip_memcpy_dtoh(got.ctypes.data, d, n * RAY.itemsize))
+        want = or_camrays(s, bx, by, samples, range(nblk)).ravel()
+        bad = np.flatnonzero(got.view(np.uint32).reshape(n, 8) != want.view(np.uint32).reshape(n, 8))
+        assert bad.size == 0, f"{bad.size} words differ, first ray {bad[0] // 8}"
+        assert (got["ox"] != got["ox"][0]).mean() > 0.99  # origins spread over the lens
+        if reference() is not None:
+            blocks = [0, nblk // 3, nblk - 1]
+            ref = ref_camrays(s, bx, by, samples, blocks)
+            npk8 = bx * by
+            for i, b in enumerate(blocks):
+                assert np.array_equal(got[b * npk8:(b + 1) * npk8].view(np.uint32),
+                                      ref[i].view(np.uint32))
+        rng = np.random.default_rng(samples + 1)
         idx = np.sort(rng.choice(n, 30000, replace=False))
         say("trace a sample of them")
         hits = s.trace(got[idx])

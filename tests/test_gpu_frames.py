@@ -17,7 +17,7 @@ import threading
 import numpy as np
 import pytest
 
-from frames_ref import NAMES, load
+from frames_ref import NAMES, load, scene_of
 from nodebleed_amd import (REND_MODE_NORMALS, REND_RCP_HOST_TABLE, REND_RCP_XEON_TABLE, Scene,
                            check, lib)
 
@@ -33,7 +33,7 @@ def test_rend_render_matches_the_reference_frames(name):
     f = load(name)
     L = lib()
     w, h, frames = int(f["width"]), int(f["height"]), int(f["frames"])
-    s = Scene(config=int(f["config"]), detail=float(f["detail"]), width=w, height=h)
+    s = scene_of(f)
     # the fixtures were written on the build host (an Intel Xeon): its
     # RCPSS/RSQRTPS tables reproduce them on any host
     check(L.rend_hip_set_rcp(REND_RCP_XEON_TABLE), "rcp")
