@@ -445,6 +445,7 @@ def guide_equal_error(a):
                   "schedule")
             check(L.rend_hip_guide_warmup(scene.rd, a.guide_warm_frames, a.guide_warm_steps),
                   "warm-up")
+            check(L.rend_hip_guide_target(scene.rd, a.guide_target), "target")
         else:
             L.rend_hip_guide_disable(scene.rd)
         curve, el, samples, k = [], 0.0, sample0, 0
@@ -512,6 +513,8 @@ def guide_equal_error(a):
                        "spp": cg[-1][1] if cg else None, "relmse": err_g, "reached": reached,
                        "curve": cg, "adam_steps": int(info[2])},
             "guide": {"mix": cfg[0], "train_stride": cfg[1], "steps_per_frame": cfg[2],
+                      "target": "next-vertex direct light" if a.guide_target else
+                                "path radiance after the vertex",
                       "warm_frames": a.guide_warm_frames,
                       "warm_steps_per_frame": a.guide_warm_steps,
                       "records_per_step": a.guide_chunk,
@@ -562,6 +565,8 @@ def main():
     ap.add_argument("--guide-chunk", type=int, default=4096,
                     help="training records per Adam step")
     ap.add_argument("--guide-warm-steps", type=int, default=64)
+    ap.add_argument("--guide-target", type=int, default=1,
+                    help="training target: 1 next vertex's direct light, 0 path radiance")
     ap.add_argument("--guide-small", action="store_true", help="480x270 instead of 1920x1080")
     ap.add_argument("--guide-profile-frames", type=int, default=0,
                     help="--mode guide: only render this many guided frames (for profiling)")

@@ -233,7 +233,7 @@ int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_
  * a single view and uses it for all of the pixel's samples (default on). */
 int          rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share);
 /* Training target of a guided vertex: the radiance its path gathers after
- * it (0, default) or only the next vertex's direct light (1: one bounce,
+ * it (0) or only the next vertex's direct light (1, default: one bounce,
  * lower variance). */
 int          rend_hip_guide_target(struct rdata *rd, int next_vertex_direct);
 /* Training objective: the KL divergence from the normalised target to the

@@ -166,12 +166,24 @@ def loss(p, rec, gmin, ginv, bsdf=0.0):
     return float(-(w * np.log(q)).sum())
 
 
-def grad(p, rec, gmin, ginv, bsdf=0.0):
-    """Analytic gradient of loss() with respect to all parameters."""
+def grad(p, rec, gmin, ginv, bsdf=0.0, emulate=False, plant=None):
+    """Analytic gradient of loss() with respect to all parameters.  With
+    emulate, the network runs at the GPU kernels' precision: FP16 hash-grid
+    features, BF16 inputs, weights and activations with wide accumulation,
+    and BF16 operands in the backward products (the output gradient, the
+    weights and the stored activations); the gradient is otherwise exact.
+    plant(go) may alter the output-layer gradient in place before it is
+    propagated: tests use it to plant mistakes the GPU comparison must catch."""
     pos, nrm = rec["pos"], rec["nrm"]
     d = rec["dir"].astype(np.float64)
-    x = encode(p, pos, nrm, gmin, ginv)
-    o, acts = forward(p, x)
+    if emulate:
+        half = lambda g: g.astype(np.float16).astype(np.float64)  # noqa: E731
+        x = encode(p, pos, nrm, gmin, ginv, grid_round=half)
+        rnd = lambda a: bf16(np.asarray(a, np.float32)).astype(np.float64)  # noqa: E731
+    else:
+        x = encode(p, pos, nrm, gmin, ginv)
+        rnd = lambda a: a  # noqa: E731
+    o, acts = forward(p, x, emulate_bf16=emulate)
     w = rec["target"].astype(np.float64)
     w = w / w.mean() / len(w)
     pi, mu, k = mixture(o)
@@ -194,6 +206,8 @@ def grad(p, rec, gmin, ginv, bsdf=0.0):
     inside = (lk > LOGK_MIN) & (lk < LOGK_MAX)
     dl = 1 / k - 2 / np.expm1(2 * k) + (c - 1)
     go[:, 4:5 * LOBES:5] = np.where(inside, -w[:, None] * r * k * dl, 0.0)
+    if plant is not None:
+        plant(go)
     g = np.zeros(NPARAM, np.float64)
     layers = weights(p)
     ofs = [(OFS_W1, OFS_B1), (OFS_W2, OFS_B2), (OFS_W3, OFS_B3), (OFS_W4, OFS_B4)]
@@ -201,9 +215,9 @@ def grad(p, rec, gmin, ginv, bsdf=0.0):
     for j in (3, 2, 1, 0):
         W, _ = layers[j]
         X = acts[j]
-        g[ofs[j][0]:ofs[j][0] + W.size] += (dz.T @ X).ravel()
+        g[ofs[j][0]:ofs[j][0] + W.size] += (rnd(dz).T @ X).ravel()
         g[ofs[j][1]:ofs[j][1] + len(layers[j][1])] += dz.sum(axis=0)
-        dx = dz @ W
+        dx = rnd(dz) @ rnd(W)
         if j > 0:
             dz = dx * (acts[j] > 0)
         else:
@@ -215,3 +229,25 @@ def grad(p, rec, gmin, ginv, bsdf=0.0):
             contrib = wc * dinput[:, 2 * l + f][:, None]
             np.add.at(g, l * TABLE * FEAT + idx.ravel() * FEAT + f, contrib.ravel())
     return g
+
+
+#: per output unit of the network's last layer (a lobe's logit, mean
+#: direction or log concentration): the GPU gradient block must agree with
+#: the BF16-emulating restatement to this cosine and norm ratio
+UNIT_COS, UNIT_RATIO = 0.99999, 5e-4
+
+
+def unit_blocks(g):
+    """The output layer's gradient per output unit: row u of W4 with bias u,
+    (GOUT, HID + 1).  Unit 5j + c is lobe j's logit (c = 0), mean direction
+    (c = 1..3) or log concentration (c = 4); units from 5 LOBES on are unused."""
+    W4 = np.asarray(g[OFS_W4:OFS_B4]).reshape(GOUT, HID)
+    return np.concatenate([W4, np.asarray(g[OFS_B4:NPARAM]).reshape(GOUT, 1)], axis=1)
+
+
+def block_agreement(got, want):
+    """Cosine and norm ratio of got against want for each output unit's block."""
+    a, b = unit_blocks(np.asarray(got, np.float64)), unit_blocks(np.asarray(want, np.float64))
+    na, nb = np.linalg.norm(a, axis=1), np.linalg.norm(b, axis=1)
+    cos = (a * b).sum(axis=1) / np.maximum(na * nb, 1e-30)
+    return cos, na / np.maximum(nb, 1e-30)

@@ -167,3 +167,53 @@ def test_sampler_is_finite_at_the_ends_of_the_unit_interval(logk):
                 v = 2.0 ** -24
                 w = 1 + np.log(v + (1 - v) * np.exp(-2 * k)) / k
                 assert abs(d[1] - w) < 1e-4 * max(1.0, abs(w)), (k, d, w)
+
+
+UNIT_COS, UNIT_RATIO = G.UNIT_COS, G.UNIT_RATIO
+
+
+def _grad_case(seed=8, n=256):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(G.NPARAM)
+    p[:G.N_GRID] = rng.uniform(-0.5, 0.5, G.N_GRID)
+    for ofs, end, fan in ((G.OFS_W1, G.OFS_B1, G.GIN), (G.OFS_W2, G.OFS_B2, G.HID),
+                          (G.OFS_W3, G.OFS_B3, G.HID), (G.OFS_W4, G.OFS_B4, G.HID)):
+        p[ofs:end] = rng.uniform(-1, 1, end - ofs) * np.sqrt(6.0 / fan)
+    p[G.OFS_B1:G.OFS_W2] = 0.05
+    p[G.OFS_B4 + 4:G.OFS_B4 + 5 * G.LOBES:5] = 1.0
+    p = p.astype(np.float32).astype(np.float64)
+    rec = np.zeros(n, G.RECORD)
+    rec["pos"] = rng.uniform(0, 1, (n, 3))
+    rec["nrm"] = uniform_sphere(n, rng)
+    rec["dir"] = uniform_sphere(n, rng)
+    rec["target"] = rng.uniform(0.1, 2.0, n)
+    gmin, ginv = np.zeros(3, np.float32), np.ones(3, np.float32) * np.float32(0.9)
+    return p, rec, gmin, ginv
+
+
+def test_unit_blocks_catch_a_wrong_lobe_gradient():
+    """The per-unit comparison the GPU gradient test applies against the
+    BF16-emulating restatement: a mistake in a single lobe's gradient -- its
+    log-concentration term 15 % off, one mean-direction component's sign, its
+    logit term dropped -- fails exactly that output unit, and the correct
+    gradient fails none.  (The same bounds hold the GPU kernel to the
+    restatement, test_gpu_guide.py.)"""
+    p, rec, gmin, ginv = _grad_case()
+    emul = G.grad(p, rec, gmin, ginv, emulate=True)
+    assert np.abs(G.unit_blocks(emul)[5 * G.LOBES:]).max() == 0.0
+
+    def kappa(go):
+        go[:, 5 * 3 + 4] *= 0.85
+
+    def direction(go):
+        go[:, 5 * 5 + 2] *= -1.0
+
+    def logit(go):
+        go[:, 5 * 1] = 0.0
+
+    for plant, unit in ((kappa, 19), (direction, 27), (logit, 5)):
+        bad = G.grad(p, rec, gmin, ginv, emulate=True, plant=plant)
+        cos, ratio = G.block_agreement(bad, emul)
+        used = np.arange(5 * G.LOBES)
+        fails = used[(cos[used] < UNIT_COS) | (np.abs(ratio[used] - 1) > UNIT_RATIO)]
+        assert fails.tolist() == [unit], (plant.__name__, fails)

@@ -74,7 +74,13 @@ def test_forward_matches_cpu(guided_scene):
     x = G.encode(p.astype(np.float64), pos, nrm, gmin, ginv, grid_round=half)
     want, _ = G.forward(p.astype(np.float64), x, emulate_bf16=True)
     err = np.abs(out - want) / (1.0 + np.abs(want))
-    assert np.median(err) < 5e-3 and err.max() < 0.1, (np.median(err), err.max())
+    say(f"  forward: error median {np.median(err):.2e}, 99.9 % {np.quantile(err, 0.999):.2e}, "
+        f"max {err.max():.2e}")
+    # the restatement rounds where the kernel rounds (FP16 grid, BF16 MFMA
+    # operands): outputs agree to FP32 accumulation order, apart from rare
+    # BF16 rounding ties decided the other way
+    assert np.median(err) < 1e-6 and np.quantile(err, 0.999) < 1e-5 and err.max() < 1e-2, (
+        np.median(err), np.quantile(err, 0.999), err.max())
 
 
 def test_mixture_kernel_matches_cpu(guided_scene):
@@ -97,11 +103,14 @@ def test_mixture_kernel_matches_cpu(guided_scene):
     assert np.isfinite(out).all()
     np.testing.assert_allclose(out[..., 0].sum(axis=1), 1.0, atol=1e-5)
     werr = np.abs(out[..., 0] - pi)
-    assert np.median(werr) < 2e-3 and werr.max() < 0.05, (np.median(werr), werr.max())
+    assert np.median(werr) < 1e-7 and werr.max() < 1e-5, (np.median(werr), werr.max())
     dots = (out[..., 1:4] * mu).sum(axis=2)
-    assert np.median(dots) > 0.9999 and dots[pi > 0.05].min() > 0.99
+    assert dots.min() > 1 - 1e-6, dots.min()
     kerr = np.abs(out[..., 4] / k - 1)
-    assert np.median(kerr) < 1e-2 and kerr.max() < 0.2, (np.median(kerr), kerr.max())
+    say(f"  mixture: weight error median {np.median(werr):.2e} max {werr.max():.2e}; "
+        f"direction dot min {dots.min():.6f}; kappa error median {np.median(kerr):.2e} "
+        f"max {kerr.max():.2e}")
+    assert np.median(kerr) < 1e-6 and kerr.max() < 1e-5, (np.median(kerr), kerr.max())
 
 
 @pytest.mark.parametrize("objective", [0, 1])
@@ -130,14 +139,37 @@ def test_gradient_matches_cpu(guided_scene, objective):
     gmin, ginv = G.scene_box(s.aabbs())
     bsdf = 0.5 if objective else 0.0
     want = G.grad(p.astype(np.float64), rec, gmin, ginv, bsdf)
-    for name, a, b in (("W1", G.OFS_W1, G.OFS_B1), ("W2", G.OFS_W2, G.OFS_B2),
-                       ("W3", G.OFS_W3, G.OFS_B3), ("W4", G.OFS_W4, G.OFS_B4),
-                       ("b4", G.OFS_B4, G.NPARAM), ("grid", 0, G.N_GRID)):
-        u, v = g[a:b].astype(np.float64), want[a:b]
+    # every output unit (a lobe's logit, mean direction or log concentration)
+    # against the restatement at the kernels' precision
+    emul = G.grad(p.astype(np.float64), rec, gmin, ginv, bsdf, emulate=True)
+    cos, ratio = G.block_agreement(g, emul)
+    used = np.arange(5 * G.LOBES)
+    say(f"  per unit vs emulated: min cosine {cos[used].min():.5f}, max |ratio - 1| "
+        f"{np.abs(ratio[used] - 1).max():.4f} (unit {used[np.argmax(np.abs(ratio[used] - 1))]})")
+    cx, rx = G.block_agreement(g, want)
+    say(f"  per unit vs exact:    min cosine {cx[used].min():.5f}, max |ratio - 1| "
+        f"{np.abs(rx[used] - 1).max():.4f}")
+    bad = used[(cos[used] < G.UNIT_COS) | (np.abs(ratio[used] - 1) > G.UNIT_RATIO)]
+    assert bad.size == 0, [(int(u), float(cos[u]), float(ratio[u])) for u in bad]
+    assert np.abs(G.unit_blocks(g)[5 * G.LOBES:]).max() == 0.0
+    # each parameter block against the restatement at the kernels' precision
+    # (and, for the record, against the exact one); the hash-grid gradient is
+    # summed by BF16 atomics in any order, hence its looser bound
+    for name, a, b, cmin, rtol in (("W1", G.OFS_W1, G.OFS_B1, 0.99999, 5e-4),
+                                   ("b1", G.OFS_B1, G.OFS_W2, 0.99999, 5e-4),
+                                   ("W2", G.OFS_W2, G.OFS_B2, 0.99999, 5e-4),
+                                   ("b2", G.OFS_B2, G.OFS_W3, 0.99999, 5e-4),
+                                   ("W3", G.OFS_W3, G.OFS_B3, 0.99999, 5e-4),
+                                   ("b3", G.OFS_B3, G.OFS_W4, 0.99999, 5e-4),
+                                   ("W4", G.OFS_W4, G.OFS_B4, 0.99999, 5e-4),
+                                   ("b4", G.OFS_B4, G.NPARAM, 0.99999, 5e-4),
+                                   ("grid", 0, G.N_GRID, 0.9999, 3e-3)):
+        u, v, x = g[a:b].astype(np.float64), emul[a:b], want[a:b]
         cos = u @ v / (np.linalg.norm(u) * np.linalg.norm(v) + 1e-30)
         ratio = np.linalg.norm(u) / (np.linalg.norm(v) + 1e-30)
-        say(f"  {name}: cosine {cos:.4f} norm ratio {ratio:.3f}")
-        assert cos > 0.97 and 0.9 < ratio < 1.1, name
+        cx = u @ x / (np.linalg.norm(u) * np.linalg.norm(x) + 1e-30)
+        say(f"  {name}: cosine {cos:.6f} norm ratio {ratio:.5f} (vs exact: cosine {cx:.4f})")
+        assert cos > cmin and abs(ratio - 1) < rtol, name
     assert abs(loss.value - G.loss(p.astype(np.float64), rec, gmin, ginv, bsdf)) < 0.05 * abs(
         loss.value) + 1e-3
 
@@ -168,8 +200,8 @@ def test_guided_rendering_is_unbiased(target, depth, share):
     blk = lambda x: x.reshape(16, 4, 16, 4, 3).mean(axis=(1, 3))  # noqa: E731
     err = rel_mse(blk(img_g), blk(img_p))
     say(f"block relMSE guided vs plain {err:.2e}, mean {img_g.mean():.4f} vs {img_p.mean():.4f}")
-    assert err < 1e-2
-    assert abs(img_g.mean() / img_p.mean() - 1) < 0.02
+    assert err < 1e-5
+    assert abs(img_g.mean() / img_p.mean() - 1) < 2e-3
 
 
 def render_spp(s, frames):
