@@ -575,7 +575,10 @@ static void m4_fromquat(float d[16], float x, float y, float z, float w)
 	d[15] = 1.0f;
 }
 
-/* Cofactor inverse; returns 1 when singular (mat4_inv, mat4.c:299-362). */
+/* Cofactor inverse (the public-domain Mesa/GLU gluInvertMatrix expansion,
+ * which the reference's mat4_inv, mat4.c:297-362, credits and uses), with
+ * the determinant's reciprocal in double as there, so instance inverses
+ * match the reference's bit for bit; returns 1 when singular. */
 static int m4_inv(float dst[16], const float m[16])
 {
 	float v[16];
