@@ -31,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from nodebleed_amd import HIT, RAY, Scene, TraceStats, check, lib  # noqa: E402
+from nodebleed_amd import HIT, RAY, Scene, TraceStats, check, lib, uneven_views  # noqa: E402
 
 W, H = 1920, 1080
 PEAK_HBM_GBS = 8000.0
@@ -332,23 +332,6 @@ def bench_blas(a):
           "host": res["host"], "device": res["device"],
           "speedup": round(res["host"]["load_s"] * 1e3 / dev, 1)}, higher=False)
     return 0
-
-
-def uneven_views(seed=1, n=32):
-    """n cameras over the terrain: resolutions from {320x240 .. 3840x2160},
-    spp uniform in 4..64, eyes on a ring above the terrain."""
-    from nodebleed_amd import VIEW
-    rng = np.random.default_rng(seed)
-    res = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
-    v = np.zeros(n, VIEW)
-    for i in range(n):
-        w, h = res[rng.integers(len(res))]
-        ang = rng.uniform(0, 2 * np.pi)
-        r = rng.uniform(0.3, 0.9)
-        eye = (r * np.cos(ang), rng.uniform(0.3, 0.9), 0.5 * r * np.sin(ang))
-        tgt = (rng.uniform(-0.3, 0.3), 0.0, rng.uniform(-0.15, 0.15))
-        v[i] = (eye, tgt, rng.uniform(30, 60), 1.0, 0.0, w, h, int(rng.integers(4, 65)))
-    return v
 
 
 def bench_views(a):

@@ -189,7 +189,10 @@ struct rend_view {
  * (captured once as a HIP graph and replayed while the view table is
  * unchanged).  d_acc holds sum(w*h*3) floats, the views' accumulators in
  * order (radiance sums; the caller clears them); sample_base offsets every
- * view's sample index so successive calls draw fresh samples.  rays (may be
+ * view's sample index so successive calls draw fresh samples.  A view's
+ * random numbers depend only on its camera, resolution and sample indices,
+ * so its image is the same whichever views share the call (two views with
+ * the same camera and resolution draw the same samples).  rays (may be
  * NULL) receives extension + shadow rays traced. */
 int          rend_hip_render_views(struct rdata *rd, const struct rend_view *views,
                                    unsigned int nviews, unsigned int sample_base,

@@ -410,6 +410,23 @@ class Scene:
             pass
 
 
+def uneven_views(seed: int = 1, n: int = 32) -> np.ndarray:
+    """The batched uneven workload: n cameras over the terrain, resolutions
+    drawn from {320x240, 640x480, 1280x720, 1920x1080, 3840x2160}, spp
+    uniform in 4..64, eyes on a ring above the terrain (VIEW records)."""
+    rng = np.random.default_rng(seed)
+    res = [(320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160)]
+    v = np.zeros(n, VIEW)
+    for i in range(n):
+        w, h = res[rng.integers(len(res))]
+        ang = rng.uniform(0, 2 * np.pi)
+        r = rng.uniform(0.3, 0.9)
+        eye = (r * np.cos(ang), rng.uniform(0.3, 0.9), 0.5 * r * np.sin(ang))
+        tgt = (rng.uniform(-0.3, 0.3), 0.0, rng.uniform(-0.15, 0.15))
+        v[i] = (eye, tgt, rng.uniform(30, 60), 1.0, 0.0, w, h, int(rng.integers(4, 65)))
+    return v
+
+
 def make_rays(o: np.ndarray, d: np.ndarray, tmax: float | np.ndarray = 3.4028235e38) -> np.ndarray:
     r = np.zeros(len(o), RAY)
     r["ox"], r["oy"], r["oz"] = o[:, 0], o[:, 1], o[:, 2]

@@ -519,6 +519,21 @@ static void view_cam(const struct rend_view *v, struct CamArgs *c)
 	c->dof = tanf_ > 0.0f;
 }
 
+/* FNV-1a over a view's camera and resolution (not its spp): the view's
+ * random stream. */
+static uint32_t view_stream(const struct rend_view *v)
+{
+	struct rend_view c = *v;
+	c.spp = 0;
+	uint32_t h = 2166136261u;
+	const unsigned char *b = (const unsigned char *)&c;
+	for (size_t i = 0; i < sizeof(c); i++) {
+		h ^= b[i];
+		h *= 16777619u;
+	}
+	return h;
+}
+
 static int views_table(const struct rend_view *in, unsigned nv, unsigned sample_base,
                        struct PtView **out)
 {
@@ -539,6 +554,7 @@ static int views_table(const struct rend_view *in, unsigned nv, unsigned sample_
 		v[i].acc_ofs = (unsigned)acc;
 		v[i].sample_base = sample_base;
 		v[i].pitch = in[i].w;
+		v[i].stream = view_stream(&in[i]);
 		first += (unsigned long long)in[i].w * in[i].h * in[i].spp;
 		acc += (unsigned long long)in[i].w * in[i].h;
 	}

@@ -110,7 +110,7 @@ __global__ void k_pt_gen(PtArgs a, const PtView *__restrict__ views,
   const uint32_t spp = v.spp;
   const uint32_t px = local / spp, s = local % spp;
   const uint32_t x = px % v.w, y = px / v.w;
-  const uint32_t key = rng_key(a.seed ^ (vi * 0x9e3779b9u), v.sample_base + s, px);
+  const uint32_t key = rng_key(a.seed ^ v.stream, v.sample_base + s, px);
   const CamArgs &c = v.cam;
   const float u0 = rng_u01(key, 0), u1 = rng_u01(key, 1);
   float sx = (float)x + (u0 - 0.5f);

@@ -11,11 +11,13 @@
 
 /* One view of a batch: camera, resolution, samples per pixel, the index of
  * its first path in the batch (prefix sum of w*h*spp), its accumulator
- * offset in pixels and the sample index of its first sample this frame. */
+ * offset in pixels, the sample index of its first sample this frame, and
+ * its random stream: a hash of the camera and resolution, so a view's
+ * samples do not depend on its place in the batch or on the other views. */
 struct PtView {
 	struct CamArgs  cam;
 	uint32_t        w, h, spp, first;
-	uint32_t        acc_ofs, sample_base, pitch, pad;
+	uint32_t        acc_ofs, sample_base, pitch, stream;
 };
 
 /* Launch constants of a frame. */
