@@ -232,8 +232,10 @@ int          rend_hip_guide_set(struct rdata *rd, float mix, unsigned int steps)
 int          rend_hip_guide_training(struct rdata *rd, unsigned int records_per_step,
                                      float lr_grid, float lr_mlp, int importance);
 /* Which vertices are guided: those at depth < depth (default 1: the
- * primary hits); share != 0 evaluates the depth-0 proposal once per pixel of
- * a single view and uses it for all of the pixel's samples (default on). */
+ * primary hits); share = k > 0 evaluates the depth-0 proposal of a single
+ * view once per run of k pixels of a row (k = 1: once per pixel, the
+ * default) and uses it for all of their samples; share = 0 evaluates it per
+ * path. */
 int          rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share);
 /* Training target of a guided vertex: the radiance its path gathers after
  * it (0) or only the next vertex's direct light (1, default: one bounce,

@@ -310,7 +310,9 @@ int rend_hip_guide_sampling(struct rdata *rd, unsigned int depth, int share)
 	if (depth < 1)
 		return rl_fail(REND_EINVAL, "guided depth %u (>= 1)", depth);
 	s->guide->depth = depth;
-	s->guide->share = share != 0;
+	/* pixels per shared depth-0 proposal: 0 off, 1 per pixel, k runs of k
+	 * consecutive pixels of a row */
+	s->guide->share = share > 0 ? (unsigned)share : 0u;
 	return 0;
 }
 

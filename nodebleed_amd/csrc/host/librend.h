@@ -71,7 +71,7 @@ struct rl_guide {
 	unsigned  stride, steps, step, last_records;
 	unsigned  chunk;                    /* records per training step */
 	unsigned  depth;                    /* vertices at depth < depth are guided */
-	int       share;                    /* one depth-0 proposal per pixel */
+	unsigned  share;                    /* pixels per shared depth-0 proposal (0: none) */
 	int       target;                   /* 1: train on next-vertex direct light */
 	int       objective;                /* 1: fit the MIS density, not the vMF alone */
 	unsigned  warm_frames, warm_steps;  /* Adam steps of the first frames after enable */

@@ -432,10 +432,14 @@ static int pt_frame(struct rl_scene *s, const struct PtView *views, unsigned nv,
 	struct PtArgs a;
 	pt_args(s, &a, nv, 0x68bc21ebu);
 	/* one view whose pixels' samples are consecutive paths of one batch:
-	 * its depth-0 proposal is evaluated once per pixel */
-	if (a.guide_mix > 0.0f && s->guide && s->guide->share && nv == 1 &&
-	    views[0].spp > 1 && t->batch % views[0].spp == 0)
-		a.guide_share = views[0].spp;
+	 * its depth-0 proposal is evaluated once per pixel, or once per run of
+	 * `share` pixels of a row */
+	if (a.guide_mix > 0.0f && s->guide && s->guide->share && nv == 1) {
+		const unsigned k = views[0].w % s->guide->share ? 1u : s->guide->share;
+		const unsigned long long run = (unsigned long long)views[0].spp * k;
+		if (run > 1 && t->batch % run == 0)
+			a.guide_share = (unsigned)run;
+	}
 	/* allocations (spill area) happen before the key and any capture */
 	unsigned g0;
 	if ((r = rl_trace_grid(1, 0, PT_RCP_IEEE, &g0)) || (r = rl_trace_grid(0, 0, PT_RCP_IEEE, &g0)))
