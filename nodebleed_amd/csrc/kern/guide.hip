@@ -37,6 +37,15 @@ namespace {
 
 constexpr int kWave = 64;
 
+// Output layer layout: the 4 lane groups of a sample hold kLpg lobes each
+// (kSlots outputs: logit, mean xyz, log concentration per lobe), spread over
+// kOutTiles 16-row MFMA tiles.
+constexpr uint32_t kLpg = GD_LOBES / 4;
+constexpr uint32_t kSlots = 5 * kLpg;
+constexpr uint32_t kOutTiles = (kSlots + 3) / 4;
+static_assert(GD_LOBES % 4 == 0 && kOutTiles <= 4 && GD_MIX <= GD_OUT,
+              "the lobes' outputs fit the 64-row output layer");
+
 __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
@@ -354,7 +363,8 @@ __global__ __launch_bounds__(256) void k_guide_tile_scan(const float *__restrict
 // block kb is unit 32 kb + 16 (j/4) + 4 (l/16) + j%4 (j = k % 8) turns the D
 // registers of tiles 2kb and 2kb+1 directly into that block's B operand:
 // activations never leave the registers.  The output layer's rows are
-// ordered so that lane group g holds lobes 2g and 2g+1 (10 outputs).
+// ordered so that lane group g holds lobes kLpg g .. kLpg g + kLpg - 1
+// (kSlots outputs).
 // The permuted bf16 copy of the weights lives at w16 + GD_NW16.
 
 // Position of hidden unit u in the permuted column order.
@@ -362,10 +372,10 @@ __host__ __device__ __forceinline__ uint32_t perm_col(uint32_t u) {
   return 32u * (u >> 5) + 8u * ((u & 15u) >> 2) + 4u * ((u & 31u) >> 4) + (u & 3u);
 }
 
-// Permuted row of output o (o < 40 used): lobe pair g = o / 10, slot
-// s = o % 10 in tile s / 4, register s % 4.
+// Permuted row of output o (o < GD_MIX used): lane group g = o / kSlots,
+// slot s = o % kSlots in tile s / 4, register s % 4.
 __host__ __device__ __forceinline__ uint32_t perm_out_row(uint32_t o) {
-  const uint32_t g = o / 10u, sl = o % 10u;
+  const uint32_t g = o / kSlots, sl = o % kSlots;
   return 16u * (sl >> 2) + 4u * g + (sl & 3u);
 }
 
@@ -694,9 +704,9 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__re
     layer64(W3, x2, s, g, h);
     x3[0] = relu_pack(h[0], h[1], mp + GD_OFS_B3, 0u, g);
     x3[1] = relu_pack(h[2], h[3], mp + GD_OFS_B3, 2u, g);
-    float o[12];
+    float o[4 * kOutTiles];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < (int)kOutTiles; t++) {
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < 2; kb++) {
@@ -707,19 +717,25 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__re
       for (int rr = 0; rr < 4; rr++) o[4 * t + rr] = acc[rr];
     }
 #pragma unroll
-    for (int sl = 0; sl < 10; sl++) o[sl] += mp[GD_OFS_B4 + 10u * g + (uint32_t)sl];
-    // ---- loss gradient for this lane's lobes 2g, 2g + 1 (slots 0..4, 5..9)
-    float mx = fmaxf(o[0], o[5]);
+    for (int sl = 0; sl < (int)kSlots; sl++) o[sl] += mp[GD_OFS_B4 + kSlots * g + (uint32_t)sl];
+    // ---- loss gradient for this lane's lobes kLpg g .. (slots 5 lb .. 5 lb + 4)
+    float mx = o[0];
+#pragma unroll
+    for (int lb = 1; lb < (int)kLpg; lb++) mx = fmaxf(mx, o[5 * lb]);
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
-    const float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
-    float esum = e0 + e1;
+    float ex[kLpg], esum = 0.0f;
+#pragma unroll
+    for (int lb = 0; lb < (int)kLpg; lb++) {
+      ex[lb] = expf(o[5 * lb] - mx);
+      esum += ex[lb];
+    }
     esum += __shfl_xor(esum, 16);
     esum += __shfl_xor(esum, 32);
-    float pi[2], mu[2][3], kap[2], cth[2], vv[2], il[2];
-    bool dirok[2];
+    float pi[kLpg], mu[kLpg][3], kap[kLpg], cth[kLpg], vv[kLpg], il[kLpg];
+    bool dirok[kLpg];
 #pragma unroll
-    for (int lb = 0; lb < 2; lb++) {
+    for (int lb = 0; lb < (int)kLpg; lb++) {
       const float *ol = &o[5 * lb];
       const float x = ol[1], y = ol[2], z = ol[3];
       const float l2 = x * x + y * y + z * z;
@@ -729,11 +745,13 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__re
       mu[lb][1] = dirok[lb] ? y * il[lb] : 0.0f;
       mu[lb][2] = dirok[lb] ? z * il[lb] : 1.0f;
       kap[lb] = expf(fminf(fmaxf(ol[4], GD_LOGK_MIN), GD_LOGK_MAX));
-      pi[lb] = (lb ? e1 : e0) / esum;
+      pi[lb] = ex[lb] / esum;
       cth[lb] = mu[lb][0] * r.dir[0] + mu[lb][1] * r.dir[1] + mu[lb][2] * r.dir[2];
       vv[lb] = gd_vmf(kap[lb], cth[lb]);
     }
-    float pd = pi[0] * vv[0] + pi[1] * vv[1];
+    float pd = 0.0f;
+#pragma unroll
+    for (int lb = 0; lb < (int)kLpg; lb++) pd += pi[lb] * vv[lb];
     pd += __shfl_xor(pd, 16);
     pd += __shfl_xor(pd, 32);
     pd = fmaxf(pd, 1e-20f);
@@ -746,12 +764,12 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__re
       qd = fmaxf((1.0f - a.train_bsdf) * pd + a.train_bsdf * cb, 1e-20f);
       wl = wgt * ((1.0f - a.train_bsdf) * pd / qd);
     }
-    float go[12];
+    float go[4 * kOutTiles];
 #pragma unroll
-    for (int k = 0; k < 12; k++) go[k] = 0.0f;
+    for (int k = 0; k < 4 * (int)kOutTiles; k++) go[k] = 0.0f;
     if (wgt > 0.0f) {
 #pragma unroll
-      for (int lb = 0; lb < 2; lb++) {
+      for (int lb = 0; lb < (int)kLpg; lb++) {
         const float rsp = pi[lb] * vv[lb] / pd;  // responsibility
         go[5 * lb] = -wl * (rsp - pi[lb]);
         if (dirok[lb]) {
@@ -769,21 +787,22 @@ __global__ __launch_bounds__(64) void k_guide_train(PtArgs a, const __half *__re
     }
     // ---- output layer: dW4, db4, dX3
 #pragma unroll
-    for (int sl = 0; sl < 10; sl++) ty[(10u * g + (uint32_t)sl) * 16u + s] = (__bf16)go[sl];
+    for (int sl = 0; sl < (int)kSlots; sl++)
+      ty[(kSlots * g + (uint32_t)sl) * 16u + s] = (__bf16)go[sl];
     put_x(tx, x3, s, g);
     wave_lds_sync();
-    dw_accum(ty, tx, 48, GD_MIX, GD_HID, S.dw + kDwW4, GD_HID);
+    dw_accum(ty, tx, 16 * (int)kOutTiles, GD_MIX, GD_HID, S.dw + kDwW4, GD_HID);
     wave_lds_sync();
 #pragma unroll
-    for (int sl = 0; sl < 10; sl++) {
+    for (int sl = 0; sl < (int)kSlots; sl++) {
       const float v = sum16(go[sl]);
-      if (s == 0) S.db[kDbB4 + 10u * g + (uint32_t)sl] += v;
+      if (s == 0) S.db[kDbB4 + kSlots * g + (uint32_t)sl] += v;
     }
     bf16x8 yb[2];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       yb[0][j] = (__bf16)go[j];
-      yb[1][j] = (__bf16)(j < 2 ? go[8 + j] : 0.0f);
+      yb[1][j] = (__bf16)(j < (int)kSlots - 8 ? go[8 + j] : 0.0f);
     }
     f32x4 d[4];
     back64(W4T, yb, s, g, 4, d);
@@ -1008,10 +1027,10 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
     layer64(W3, xb, s, g, h);
     xb[0] = relu_pack(h[0], h[1], master + GD_OFS_B3, 0u, g);
     xb[1] = relu_pack(h[2], h[3], master + GD_OFS_B3, 2u, g);
-    // output layer: tiles 0..2 hold this lane group's 10 outputs
-    float o[12];
+    // output layer: tiles 0..kOutTiles-1 hold this lane group's kSlots outputs
+    float o[4 * kOutTiles];
 #pragma unroll
-    for (int t = 0; t < 3; t++) {
+    for (int t = 0; t < (int)kOutTiles; t++) {
       f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kb = 0; kb < 2; kb++) {
@@ -1022,35 +1041,43 @@ __global__ __launch_bounds__(256, 4) void k_guide_mix(PtArgs a, const __half *__
       for (int r = 0; r < 4; r++) o[4 * t + r] = acc[r];
     }
 #pragma unroll
-    for (int sl = 0; sl < 10; sl++) o[sl] += master[GD_OFS_B4 + 10u * g + (uint32_t)sl];
-    // softmax over the 8 lobes: the sample's 4 lanes are s, s+16, s+32, s+48
-    float mx = fmaxf(o[0], o[5]);
+    for (int sl = 0; sl < (int)kSlots; sl++)
+      o[sl] += master[GD_OFS_B4 + kSlots * g + (uint32_t)sl];
+    // softmax over the lobes: the sample's 4 lanes are s, s+16, s+32, s+48
+    float mx = o[0];
+#pragma unroll
+    for (int lb = 1; lb < (int)kLpg; lb++) mx = fmaxf(mx, o[5 * lb]);
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
-    const float e0 = expf(o[0] - mx), e1 = expf(o[5] - mx);
-    float sum = e0 + e1;
+    float ex[kLpg], sum = 0.0f;
+#pragma unroll
+    for (int lb = 0; lb < (int)kLpg; lb++) {
+      ex[lb] = expf(o[5 * lb] - mx);
+      sum += ex[lb];
+    }
     sum += __shfl_xor(sum, 16);
     sum += __shfl_xor(sum, 32);
-    float rec[10];
+    float rec[kSlots];
 #pragma unroll
-    for (int lb = 0; lb < 2; lb++) {
+    for (int lb = 0; lb < (int)kLpg; lb++) {
       const float *ol = &o[5 * lb];
       float x = ol[1], y = ol[2], z = ol[3];
       const float l2 = x * x + y * y + z * z;
       if (l2 < 1e-12f) { x = 0.0f; y = 0.0f; z = 1.0f; }
       else { const float il = 1.0f / sqrtf(l2); x *= il; y *= il; z *= il; }
       const float lk = fminf(fmaxf(ol[4], GD_LOGK_MIN), GD_LOGK_MAX);
-      rec[5 * lb] = (lb ? e1 : e0) / sum;
+      rec[5 * lb] = ex[lb] / sum;
       rec[5 * lb + 1] = x;
       rec[5 * lb + 2] = y;
       rec[5 * lb + 3] = z;
       rec[5 * lb + 4] = expf(lk);
     }
-    if (!ok) rec[0] = rec[5] = -1.0f;  // empty proposal
+    if (!ok)  // empty proposal
+      for (int lb = 0; lb < (int)kLpg; lb++) rec[5 * lb] = -1.0f;
     if (i < n) {
-      float2 *dst = (float2 *)&mix[(size_t)i * GD_MIX + 10u * g];
+      float *dst = &mix[(size_t)i * GD_MIX + kSlots * g];
 #pragma unroll
-      for (int k = 0; k < 5; k++) dst[k] = make_float2(rec[2 * k], rec[2 * k + 1]);
+      for (int k = 0; k < (int)kSlots; k++) dst[k] = rec[k];
     }
   }
 }
