@@ -590,6 +590,12 @@ def main():
 
     st_d, st_p = TraceStats(), TraceStats()
     check(L.rend_hip_trace(scene.rd, d_diff, d_dhit, n, stream, C.byref(st_d)), "count")
+    lanes = (C.c_ulonglong * 4)()
+    check(L.rend_hip_trace_lanes(lanes), "lane counters")
+    wave = {"iterations": int(lanes[0]),
+            "lanes_per_iteration": round(lanes[1] / max(lanes[0], 1), 2),
+            "leaf_steps": int(lanes[2]), "setup_steps": int(lanes[3])}
+    log(f"diffuse launch wave loop: {wave}")
     check(L.rend_hip_trace(scene.rd, d_prim, d_phit, n, stream, C.byref(st_p)), "count")
     phits = np.zeros(n, HIT)
     check(L.rend_hip_memcpy_dtoh(phits.ctypes.data, d_phit, n * HIT.itemsize), "dtoh")
@@ -677,6 +683,7 @@ def main():
                                "leaves": round(st_d.leaves / n, 3),
                                "insts": round(st_d.insts / n, 3)},
             "device": f"{name.value.decode()} ({cus.value} CUs)",
+            "wave_loop": wave,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,

@@ -116,32 +116,32 @@ __device__ __noinline__ void report_error(uint32_t *diag, uint32_t code, const r
 // finite (no NaN), so IEEE max/min (v_max3/v_min3) make the same decisions
 // as the reference's x86 MAXPS/MINPS operand rules.
 template <bool FAST, bool HOIST>
-__device__ __forceinline__ uint32_t slab8(const dnode *ndp, const RayS &cr, float t,
-                                          float tmn[8], uint32_t ch[8], uint32_t &invw) {
-  // nodes sit at 256-byte aligned arena offsets
-  const dnode *nd = (const dnode *)__builtin_assume_aligned(ndp, 256);
-  // near/far bound rows of each axis (32-byte rows: min, max per axis)
-  const char *nb = (const char *)nd;
+__device__ __forceinline__ uint32_t slab8(const uint8_t *arena, uint32_t nofs, const RayS &cr,
+                                          float t, float tmn[8], uint32_t ch[8],
+                                          uint32_t &invw) {
+  // The node sits at the 32-bit arena offset nofs (256-byte aligned).  Every
+  // address is the arena base plus a 32-bit offset, which the loads take as
+  // a scalar base and a 32-bit vector offset (no 64-bit address arithmetic
+  // per lane).  Near/far bound rows of each axis: 32-byte rows (min, max).
   const uint32_t ax = (cr.oct & 1u) ? 0u : 32u, ay = (cr.oct & 2u) ? 64u : 96u,
                  az = (cr.oct & 4u) ? 128u : 160u;
-  const float *nx = (const float *)__builtin_assume_aligned(nb + ax, 32);
-  const float *fx = (const float *)__builtin_assume_aligned(nb + (ax ^ 32u), 32);
-  const float *ny = (const float *)__builtin_assume_aligned(nb + ay, 32);
-  const float *fy = (const float *)__builtin_assume_aligned(nb + (ay ^ 32u), 32);
-  const float *nz = (const float *)__builtin_assume_aligned(nb + az, 32);
-  const float *fz = (const float *)__builtin_assume_aligned(nb + (az ^ 32u), 32);
+  auto row = [&](uint32_t o) { return (const float4 *)(arena + (uint32_t)(nofs + o)); };
   float nxv[8], fxv[8], nyv[8], fyv[8], nzv[8], fzv[8];
+  const float4 *nx = row(ax), *fx = row(ax ^ 32u), *ny = row(ay), *fy = row(ay ^ 32u),
+               *nz = row(az), *fz = row(az ^ 32u);
+  const uint4 *cw = (const uint4 *)(arena + (uint32_t)(nofs + (uint32_t)offsetof(dnode, child)));
 #pragma unroll
-  for (int k = 0; k < 8; k += 4) {
-    *(float4 *)&nxv[k] = *(const float4 *)&nx[k];
-    *(float4 *)&fxv[k] = *(const float4 *)&fx[k];
-    *(float4 *)&nyv[k] = *(const float4 *)&ny[k];
-    *(float4 *)&fyv[k] = *(const float4 *)&fy[k];
-    *(float4 *)&nzv[k] = *(const float4 *)&nz[k];
-    *(float4 *)&fzv[k] = *(const float4 *)&fz[k];
-    *(uint4 *)&ch[k] = *(const uint4 *)&nd->child[k];
+  for (int k = 0; k < 2; k++) {
+    *(float4 *)&nxv[4 * k] = nx[k];
+    *(float4 *)&fxv[4 * k] = fx[k];
+    *(float4 *)&nyv[4 * k] = ny[k];
+    *(float4 *)&fyv[4 * k] = fy[k];
+    *(float4 *)&nzv[4 * k] = nz[k];
+    *(float4 *)&fzv[4 * k] = fz[k];
+    *(uint4 *)&ch[4 * k] = cw[k];
   }
-  invw = nd->inv[cr.oct];
+  invw = *(const uint32_t *)(arena + (uint32_t)(nofs + 4u * cr.oct +
+                                                (uint32_t)offsetof(dnode, inv)));
   uint32_t hmr = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -189,8 +189,13 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   __shared__ uint16_t s_tab[RCP_IEEE ? 2 : 4096];
   if (n_dev) n = *n_dev;  // queue length produced by an earlier kernel
   // stack rows [0, DEPTH) per lane, plus one junk row that branch-free
-  // pushes write for children they do not keep
-  __shared__ uint2 s_stack[DEPTH + 1][REND_TRACE_BLOCK];
+  // pushes write for children they do not keep.  Entry words (child offset)
+  // and entry distances are separate arrays: a push then stores two
+  // independent registers, where a packed pair would first be copied into
+  // adjacent registers.  The any-hit pass keeps no distances.
+  // One array, so a distance sits at a constant offset from its entry
+  // word and both stores share one address.
+  __shared__ uint32_t s_stk[CLOSEST ? 2 : 1][DEPTH + 1][REND_TRACE_BLOCK];
   if (!RCP_IEEE) {
     const bool xeon = (flags & RK_TRACE_XEON) != 0u;
     for (uint32_t i = threadIdx.x; i < 4096; i += REND_TRACE_BLOCK)
@@ -207,9 +212,10 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   // LDS is always addressed through a clamped row (deep entries touch the
   // junk row) so LDS and spill accesses never share a generic pointer
   auto put = [&](uint32_t i, uint32_t o, float d) {
-    const uint2 e = make_uint2(o, __float_as_uint(d));
-    s_stack[i < (uint32_t)DEPTH ? i : (uint32_t)DEPTH][tid] = e;
-    if (i >= (uint32_t)DEPTH) myspill[(i - DEPTH) * sstride] = e;
+    const uint32_t row = i < (uint32_t)DEPTH ? i : (uint32_t)DEPTH;
+    s_stk[0][row][tid] = o;
+    if (CLOSEST) s_stk[CLOSEST ? 1 : 0][row][tid] = __float_as_uint(d);
+    if (i >= (uint32_t)DEPTH) myspill[(i - DEPTH) * sstride] = make_uint2(o, __float_as_uint(d));
   };
 
   // Lane states.  A lane waits in its state until the wave runs that
@@ -260,21 +266,21 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
 
     bool need_pop = false;
     bool done = false;
+    uint32_t err = 0;  // error code of a lane that ends on an error
 
     // ---- node step -------------------------------------------------------
     const uint64_t m_node = __ballot(state == ST_NODE);
     if (m_node && state == ST_NODE) {
       if (++iter > max_iter) {
-        report_error(diag, RK_ERR_ITER, rays, rid, iter);
+        err = RK_ERR_ITER;
         done = true;
       } else {
-        const dnode *nd = (const dnode *)(arena + (base + ofs));
         float tmn[8];
         uint32_t ch[8], invw;
         // the path is chosen before the node is read, so each is one block
         const uint32_t hmr = __ballot(!(cr.fin && t == t) && state == ST_NODE) == 0
-                                 ? slab8<true, RCP_IEEE>(nd, cr, t, tmn, ch, invw)
-                                 : slab8<false, RCP_IEEE>(nd, cr, t, tmn, ch, invw);
+                                 ? slab8<true, RCP_IEEE>(arena, base + ofs, cr, t, tmn, ch, invw)
+                                 : slab8<false, RCP_IEEE>(arena, base + ofs, cr, t, tmn, ch, invw);
         if (COUNT) c_nodes++;
 
         const uint32_t hitcnt = __builtin_popcount(hmr);
@@ -283,7 +289,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         } else if (sp + hitcnt > REND_STACK_MAX) {
           // The reference asserts here (rend.c:834); report instead of
           // writing past the stack.
-          report_error(diag, RK_ERR_STACK, rays, rid, iter);
+          err = RK_ERR_STACK;
           done = true;
         } else {
           // Highest rank among hits (nearest) is the next node, the others
@@ -303,8 +309,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
               const bool hit = (keym >> key) & 1u;
               const uint32_t pos = __builtin_popcount(keym & ((1u << key) - 1u));
               const uint32_t slot = (hit && key != top) ? sp + pos : (uint32_t)DEPTH;
-              s_stack[slot][tid] =
-                  make_uint2(ch[c], CLOSEST ? __float_as_uint(tmn[c]) : 0u);
+              s_stk[0][slot][tid] = ch[c];
+              if (CLOSEST) s_stk[CLOSEST ? 1 : 0][slot][tid] = __float_as_uint(tmn[c]);
               next = key == top ? ch[c] : next;
             }
           } else {
@@ -339,7 +345,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       if (COUNT) c_leafx++;
       if (state == ST_LEAF && !need_pop && !done) {
         if (++iter > max_iter) {
-          report_error(diag, RK_ERR_ITER, rays, rid, iter);
+          err = RK_ERR_ITER;
           done = true;
         } else {
           const dleaf *lf = (const dleaf *)(arena + (base + (ofs & ~DNODE_LEAF)));
@@ -416,41 +422,47 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     if (m_setup && ((uint32_t)__popcll(m_setup) >= kStepBatch || n_node < kStepBatch)) {
       if (COUNT) c_setupx++;
       if (in_setup && !need_pop && !done) {
-        if (state != ST_SETUP_INST) {
-          if (state == ST_SETUP_NEW) { t = tu; tu = 0.0f; }
-          ray_setup(cr, wo[0], wo[1], wo[2], wd[0], wd[1], wd[2]);
-          state = ST_NODE;
-        } else if (++iter > max_iter) {
-          report_error(diag, RK_ERR_ITER, rays, rid, iter);
-          done = true;
-        } else {
-          // TLAS leaf: enter the instance's BLAS with the transformed ray.
-          instid = ofs & ~DNODE_LEAF;
-          const dinst *in = insts + instid;
-          const float4 m0 = *(const float4 *)&in->inv[0];
-          const float4 m1 = *(const float4 *)&in->inv[4];
-          const float4 m2 = *(const float4 *)&in->inv[8];
-          const uint64_t blas = in->blas;
-          // wait for the BLAS offset here, beside the matrix rows: left
-          // outstanding, the next node step (which needs it for its address)
-          // would have to drain the whole vector-memory counter, hit stores
-          // and refill loads included
-          asm volatile("" ::"v"(blas));
-          if (COUNT) c_insts++;
-          const float ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
-          const float oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
-          const float oz = m2.x * wo[0] + m2.y * wo[1] + m2.z * wo[2] + m2.w;
-          const float dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
-          const float dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
-          const float dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
-          if (flags & RK_TRACE_NUDGE)
-            ray_setup(cr, ox, oy, oz, nudge0(dx), nudge0(dy), nudge0(dz));
-          else
-            ray_setup(cr, ox, oy, oz, dx, dy, dz);
-          in_blas = true;
-          blas_sp = sp;
-          base = (uint32_t)blas;
-          ofs = 0;
+        // One slab set-up (three IEEE divisions) serves both kinds: the
+        // world ray as it is, or the world ray in the instance's space.
+        float ox = wo[0], oy = wo[1], oz = wo[2], dx = wd[0], dy = wd[1], dz = wd[2];
+        if (state == ST_SETUP_NEW) { t = tu; tu = 0.0f; }
+        if (state == ST_SETUP_INST) {
+          if (++iter > max_iter) {
+            err = RK_ERR_ITER;
+            done = true;
+          } else {
+            // TLAS leaf: enter the instance's BLAS with the transformed ray.
+            instid = ofs & ~DNODE_LEAF;
+            const dinst *in = insts + instid;
+            const float4 m0 = *(const float4 *)&in->inv[0];
+            const float4 m1 = *(const float4 *)&in->inv[4];
+            const float4 m2 = *(const float4 *)&in->inv[8];
+            const uint64_t blas = in->blas;
+            // wait for the BLAS offset here, beside the matrix rows: left
+            // outstanding, the next node step (which needs it for its
+            // address) would have to drain the whole vector-memory counter,
+            // hit stores and refill loads included
+            asm volatile("" ::"v"(blas));
+            if (COUNT) c_insts++;
+            ox = m0.x * wo[0] + m0.y * wo[1] + m0.z * wo[2] + m0.w;
+            oy = m1.x * wo[0] + m1.y * wo[1] + m1.z * wo[2] + m1.w;
+            oz = m2.x * wo[0] + m2.y * wo[1] + m2.z * wo[2] + m2.w;
+            dx = m0.x * wd[0] + m0.y * wd[1] + m0.z * wd[2];
+            dy = m1.x * wd[0] + m1.y * wd[1] + m1.z * wd[2];
+            dz = m2.x * wd[0] + m2.y * wd[1] + m2.z * wd[2];
+            if (flags & RK_TRACE_NUDGE) {
+              dx = nudge0(dx);
+              dy = nudge0(dy);
+              dz = nudge0(dz);
+            }
+            in_blas = true;
+            blas_sp = sp;
+            base = (uint32_t)blas;
+            ofs = 0;
+          }
+        }
+        if (!done) {
+          ray_setup(cr, ox, oy, oz, dx, dy, dz);
           state = ST_NODE;
         }
       }
@@ -469,7 +481,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         }
         if (sp == 0) { done = true; break; }
         --sp;
-        uint2 e = s_stack[sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH][tid];
+        const uint32_t row = sp < (uint32_t)DEPTH ? sp : (uint32_t)DEPTH;
+        uint2 e = make_uint2(s_stk[0][row][tid], CLOSEST ? s_stk[CLOSEST ? 1 : 0][row][tid] : 0u);
         if (sp >= (uint32_t)DEPTH) {
           // volatile keeps the compiler from merging this load with the LDS
           // one; the global address space keeps it a global (not flat)
@@ -489,6 +502,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     }
 
     if (done) {
+      if (err) report_error(diag, err, rays, rid, iter);
       if (CLOSEST) {
         rhit h; h.t = t; h.u = tu; h.v = tv; h.id = hid;
         *(float4 *)&hits[rid] = *(const float4 *)&h;
