@@ -243,7 +243,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   uint32_t rid = 0;
   RayS cr;                    // ray of the current tree (world or instance)
   float wo[3] = {0, 0, 0}, wd[3] = {0, 0, 0};  // world ray
-  float t = 0.0f, tu = 0.0f, tv = 0.0f;
+  float t = 0.0f, tu = 0.0f, tv = 0.0f, tnew = 0.0f;
   uint32_t hid = 0xffffffffu;
   uint32_t ofs = 0, sp = 0, blas_sp = 0, instid = 0, iter = 0;
   uint32_t base = (uint32_t)tlas;  // arena offset of the current tree (< 4 GiB)
@@ -277,6 +277,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
       } else {
         float tmn[8];
         uint32_t ch[8], invw;
+        constexpr bool HOIST_CH = RCP_IEEE;  // slab8 already consumes the child words
         // the path is chosen before the node is read, so each is one block
         const uint32_t hmr = __ballot(!(cr.fin && t == t) && state == ST_NODE) == 0
                                  ? slab8<true, RCP_IEEE>(arena, base + ofs, cr, t, tmn, ch, invw)
@@ -331,6 +332,14 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
           ofs = next;
           state = classify(next);
         }
+        // Waves whose lanes all missed never read the child words; left in
+        // flight, their loads would make the next node step, which may reuse
+        // those registers, first wait for every outstanding memory access
+        // (this iteration's refill loads included).  Consumed here, they
+        // arrive with the bounds and cost no wait of their own.
+        if (!HOIST_CH)
+          asm volatile("" ::"v"(ch[0]), "v"(ch[1]), "v"(ch[2]), "v"(ch[3]), "v"(ch[4]),
+                       "v"(ch[5]), "v"(ch[6]), "v"(ch[7]));
       }
     }
 
@@ -425,7 +434,10 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         // One slab set-up (three IEEE divisions) serves both kinds: the
         // world ray as it is, or the world ray in the instance's space.
         float ox = wo[0], oy = wo[1], oz = wo[2], dx = wd[0], dy = wd[1], dz = wd[2];
-        if (state == ST_SETUP_NEW) { t = tu; tu = 0.0f; }
+        if (state == ST_SETUP_NEW) {
+          t = RCP_IEEE ? tu : tnew;
+          tu = 0.0f;
+        }
         if (state == ST_SETUP_INST) {
           if (++iter > max_iter) {
             err = RK_ERR_ITER;
@@ -542,11 +554,21 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
         const uint32_t r = lnext + below;
         if (below < take) {
           rid = r;
-          const float4 a = *(const float4 *)&rays[r].ox;
-          const float4 b = *(const float4 *)&rays[r].dx;
-          wo[0] = a.x; wo[1] = a.y; wo[2] = a.z;
-          wd[0] = b.x; wd[1] = b.y; wd[2] = b.z;
-          tu = a.w;  // tmax, moved to t by the SETUP_NEW step
+          // Origin, direction and tmax should land in the registers the
+          // loop carries them in.  Read as one 16-byte load with the origin,
+          // tmax is copied out of it, and that copy waits for the record
+          // right here; a load of its own (an opaque offset keeps the loads
+          // apart) avoids the wait.  The IEEE-reciprocal kernels, at their
+          // register limit, would spill for it and keep the single load.
+          const float *rr = (const float *)&rays[r];
+          uint32_t o3 = 3u;
+          if (!RCP_IEEE) asm volatile("" : "+v"(o3));
+          wo[0] = rr[0]; wo[1] = rr[1]; wo[2] = rr[2];
+          wd[0] = rr[4]; wd[1] = rr[5]; wd[2] = rr[6];
+          // tmax, moved to t by the SETUP_NEW step; its own register keeps
+          // the hit store (t, u, v, id) from waiting for this load
+          if (RCP_IEEE) tu = rr[o3];
+          else tnew = rr[o3];
           tv = 0.0f;
           hid = 0xffffffffu;
           ofs = 0; sp = 0; blas_sp = 0; iter = 0;
