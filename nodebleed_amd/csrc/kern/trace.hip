@@ -110,6 +110,24 @@ __device__ __noinline__ void report_error(uint32_t *diag, uint32_t code, const r
   }
 }
 
+// One step of a quad reduction to the nearest hit: exchange with the lane
+// given by the DPP quad permutation CTRL; hi names the lane holding the
+// higher triangle index of the pair, which wins equal distances.  No value
+// is NaN (a miss carries FLT_MAX), so the comparisons are plain.
+template <int CTRL>
+__device__ __forceinline__ void quad_nearest(uint32_t hi, float &tb, float &u, float &v,
+                                             uint32_t &id) {
+  const float pt = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(tb), CTRL, 0xf, 0xf, false));
+  const float pu = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(u), CTRL, 0xf, 0xf, false));
+  const float pv = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+  const uint32_t pid = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)id, CTRL, 0xf, 0xf, false);
+  const bool take = hi ? (pt < tb) : (pt <= tb);
+  tb = take ? pt : tb;
+  u = take ? pu : u;
+  v = take ? pv : v;
+  id = take ? pid : id;
+}
+
 // Loads a node and runs the slab test of its 8 children: entry distances
 // into tmn, child words into ch, the octant's rank word into invw; returns
 // the hit mask indexed by rank.  FAST: every slab distance is known to be
@@ -234,6 +252,13 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
                     ST_SETUP_WORLD = 4, ST_SETUP_NEW = 5 };
   // leaf and setup steps run for at least this many waiting lanes, or once
   // fewer lanes than this are left at interior nodes
+  // Leaf steps spread the waiting lanes' triangles over the wave (below) in
+  // the host-table kernels.  The IEEE-reciprocal kernels (the path
+  // tracer's) load each node's child words with its bounds; with the
+  // exchange's registers on top of those the closest-hit one spills, and
+  // without the early child load path-traced frames were no faster, so they
+  // test each leaf in its own lane.
+  constexpr bool SPREAD_LEAF = !RCP_IEEE;
   const uint32_t kStepBatch = (flags >> 8) & 0xffu ? (flags >> 8) & 0xffu : RK_STEP_BATCH;
   uint32_t state = ST_IDLE;
   bool exhausted = false;  // wave-uniform
@@ -350,7 +375,111 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
     // pop (most pops lead to another interior node)
     const uint64_t m_node2 = __ballot((state == ST_NODE || need_pop) && !done);
     const uint32_t n_node = (uint32_t)__popcll(m_node2);
-    if (m_leaf && ((uint32_t)__popcll(m_leaf) >= kStepBatch || n_node < kStepBatch)) {
+    const bool leaf_step =
+        m_leaf && ((uint32_t)__popcll(m_leaf) >= kStepBatch || n_node < kStepBatch);
+    if (SPREAD_LEAF && leaf_step) {
+      // The waiting lanes' leaves are spread over the whole wave, one
+      // triangle per lane: the q-th waiting lane's four triangles go to lanes
+      // 4q .. 4q+3 (16 leaves per pass), each lane runs the reference's
+      // intersection arithmetic for its triangle with the owner's ray, and
+      // a quad reduction picks the nearest hit with the reference's tie rule
+      // (the highest-indexed of the equal minima).  Every lane of the wave
+      // takes part in the exchanges, so this block is wave-uniform.
+      if (COUNT) c_leafx++;
+      bool part = state == ST_LEAF && !need_pop && !done;
+      if (part && ++iter > max_iter) {
+        err = RK_ERR_ITER;
+        done = true;
+        part = false;
+      }
+      if (COUNT && part) c_leaves++;
+      // lane-derived values are recomputed here from an opaque copy of the
+      // lane index: hoisted out of the loop they would hold registers
+      // through the node step
+      uint32_t lid = lane;
+      asm volatile("" : "+v"(lid));
+      const uint64_t mp = __ballot(part);
+      const uint32_t nleaf = (uint32_t)__popcll(mp);
+      const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(mp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mp, 0u));
+      const uint32_t other = lid - rank;  // non-waiting lanes below this one
+      // lane q of tab holds the lane id of the q-th waiting lane: waiting
+      // lanes send their id to lane rank, the others to nleaf + other, so
+      // every lane receives exactly one value
+      const uint32_t tab = (uint32_t)__builtin_amdgcn_ds_permute(
+          (int)((part ? rank : nleaf + other) << 2), (int)lid);
+      const uint32_t lofs = base + (ofs & ~DNODE_LEAF);
+      const uint32_t k = lid & 3u;
+      for (uint32_t q0 = 0; q0 < nleaf; q0 += 16u) {
+        const uint32_t q = q0 + (lid >> 2);
+        const bool act = q < nleaf;
+        const int src = (int)__shfl((int)tab, act ? (int)q : 0);
+        const float ox = __shfl(cr.ox, src), oy = __shfl(cr.oy, src), oz = __shfl(cr.oz, src);
+        const float dx = __shfl(cr.dx, src), dy = __shfl(cr.dy, src), dz = __shfl(cr.dz, src);
+        const float tcur = __shfl(t, src);
+        const uint32_t lo = (uint32_t)__shfl((int)lofs, src);
+        float tb = FLT_MAXF, u = 0.0f, v = 0.0f;
+        uint32_t id = 0;
+        bool ok = false;
+        if (act) {
+          // this lane's column of the SoA leaf: the arena base plus a 32-bit
+          // offset (scalar base, vector offset, constant field offsets)
+          const float *lf = (const float *)(arena + (uint32_t)(lo + 4u * k));
+          const float V0x = lf[0], V0y = lf[4], V0z = lf[8];
+          const float E0x = lf[12], E0y = lf[16], E0z = lf[20];
+          const float E1x = lf[24], E1y = lf[28], E1z = lf[32];
+          if (CLOSEST) id = ((const uint32_t *)lf)[36];
+          const float pvx = __builtin_fmaf(dy, E1z, -(dz * E1y));
+          const float pvy = __builtin_fmaf(dz, E1x, -(dx * E1z));
+          const float pvz = __builtin_fmaf(dx, E1y, -(dy * E1x));
+          const float tvx = ox - V0x;
+          const float tvy = oy - V0y;
+          const float tvz = oz - V0z;
+          const float det = __builtin_fmaf(E0x, pvx, __builtin_fmaf(E0y, pvy, E0z * pvz));
+          const float qvx = __builtin_fmaf(tvy, E0z, -(tvz * E0y));
+          const float qvy = __builtin_fmaf(tvz, E0x, -(tvx * E0z));
+          const float qvz = __builtin_fmaf(tvx, E0y, -(tvy * E0x));
+          const float idet = RCP_IEEE ? 1.0f / det : rcp_table(det, s_tab);
+          u = idet * __builtin_fmaf(tvx, pvx, __builtin_fmaf(tvy, pvy, tvz * pvz));
+          v = idet * __builtin_fmaf(dx, qvx, __builtin_fmaf(dy, qvy, dz * qvz));
+          const float tk = idet * __builtin_fmaf(E1x, qvx, __builtin_fmaf(E1y, qvy, E1z * qvz));
+          ok = (u >= 0.0f) & (v >= 0.0f) & ((u + v) <= 1.0f) & (tk > 0.0f) & (tk < tcur);
+          tb = ok ? tk : FLT_MAXF;
+        }
+        const uint64_t okm = __ballot(ok);
+        if (CLOSEST) {
+          // nearest of the quad; on equal distances the higher triangle
+          // index wins (the reference keeps the last of the equal minima)
+          quad_nearest<0xB1>(k & 1u, tb, u, v, id);
+          quad_nearest<0x4E>(k & 2u, tb, u, v, id);
+        }
+        // owners of this pass read their quad's result from its first lane
+        const bool mine = part && rank >= q0 && rank < q0 + 16u;
+        const uint32_t lead = mine ? (rank - q0) << 2 : 0u;
+        const bool qhit = ((okm >> lead) & 0xfull) != 0ull;
+        // (a lane owns a leaf in one pass only, so its t can change here)
+        if (CLOSEST) {
+          const float qt = __shfl(tb, (int)lead), qu = __shfl(u, (int)lead),
+                      qv = __shfl(v, (int)lead);
+          const uint32_t qid = (uint32_t)__shfl((int)id, (int)lead);
+          if (mine) {
+            if (qhit) {
+              t = qt;
+              tu = qu;
+              tv = qv;
+              hid = (qid << 12) | instid;
+            }
+            need_pop = true;
+          }
+        } else if (mine) {
+          if (qhit) { done = true; occluded = true; }
+          else need_pop = true;
+        }
+      }
+    }
+
+    if (!SPREAD_LEAF && leaf_step) {
+      // each waiting lane tests its own leaf's four triangles
       if (COUNT) c_leafx++;
       if (state == ST_LEAF && !need_pop && !done) {
         if (++iter > max_iter) {
