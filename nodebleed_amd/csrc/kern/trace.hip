@@ -245,13 +245,12 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   //   SETUP_WORLD the world ray's slab constants are needed (a TLAS node
   //               popped after a BLAS)
   //   SETUP_NEW   a new ray at the TLAS root: the refill loaded its record
-  //               into the world-ray registers and its tmax into tu (both
-  //               unread by the node step, which so never waits for that
-  //               load); the next setup step sets t and the slab constants
+  //               into the world-ray registers and its tmax into a register
+  //               of its own (both unread by the node step, which so never
+  //               waits for that load); the next setup step sets t and the
+  //               slab constants
   enum : uint32_t { ST_IDLE = 0, ST_NODE = 1, ST_LEAF = 2, ST_SETUP_INST = 3,
                     ST_SETUP_WORLD = 4, ST_SETUP_NEW = 5 };
-  // leaf and setup steps run for at least this many waiting lanes, or once
-  // fewer lanes than this are left at interior nodes
   // Leaf steps spread the waiting lanes' triangles over the wave (below) in
   // the host-table kernels.  The IEEE-reciprocal kernels (the path
   // tracer's) load each node's child words with its bounds; with the
@@ -259,6 +258,8 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
   // without the early child load path-traced frames were no faster, so they
   // test each leaf in its own lane.
   constexpr bool SPREAD_LEAF = !RCP_IEEE;
+  // leaf and setup steps run for at least this many waiting lanes, or once
+  // fewer lanes than this are left at interior nodes
   const uint32_t kStepBatch = (flags >> 8) & 0xffu ? (flags >> 8) & 0xffu : RK_STEP_BATCH;
   uint32_t state = ST_IDLE;
   bool exhausted = false;  // wave-uniform
@@ -655,7 +656,7 @@ void k_trace(const uint8_t *__restrict__ arena, uint64_t tlas,
 
     // ---- refill idle lanes from the wave's claimed chunk ------------------
     // At the end of the loop body: a new ray's record is loaded here and its
-    // slab constants computed by the next iteration's SETUP_WORLD step, after
+    // slab constants computed by the next iteration's SETUP_NEW step, after
     // the node step, so the load's latency overlaps the other lanes' work.
     const uint64_t idle = __ballot(state == ST_IDLE);
     const uint32_t nidle = (uint32_t)__popcll(idle);
