@@ -518,8 +518,12 @@ def bench_guide(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # defaults: the headline (trace) line times 100 launches (~36 ms of GPU
+    # time) after 10 warm-up ones, since with 20 after 3 its mean still
+    # includes launches at lower clocks (~3 % slower); the other modes, whose
+    # steps are whole frames or calls, time 20 after 3
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
@@ -560,6 +564,10 @@ def main():
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch path-tracing frames directly instead of replaying HIP graphs")
     a = ap.parse_args()
+    if a.steps is None:
+        a.steps = 100 if a.mode == "trace" else 20
+    if a.warmup is None:
+        a.warmup = 10 if a.mode == "trace" else 3
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path", file=sys.stderr)
