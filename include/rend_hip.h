@@ -144,8 +144,8 @@ int          rend_hip_set_step_batch(unsigned int lanes);
  * set-up steps run. */
 int          rend_hip_trace_lanes(unsigned long long out[4]);
 
-/* Traversal kernel variant (0..3): LDS stack depth / waves per SIMD of
- * 16/4, 12/5, 10/6 and 8/8 (deeper entries spill to HBM).  Results are
+/* Traversal kernel variant (0..3): LDS stack entries / waves per SIMD of
+ * 15/4, 11/5, 8/6 and 5/8 (deeper entries go to a global spill area).  Results are
  * identical for every variant; only speed differs.  Default 0. */
 int          rend_hip_set_trace_variant(int variant);
 
