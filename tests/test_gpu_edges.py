@@ -1,13 +1,14 @@
 """Edge cases of the traversal boundary on the GPU: empty and ragged ray
 counts (not multiples of a wave, a workgroup or a claim chunk), empty ray
 intervals, and a ray count well above the benchmark's, each against the
-oracle (bit-exact hits, equal any-hit flags)."""
+oracle (bit-exact hits, equal any-hit flags); and the iteration-limit
+error path, forced from the host with a small limit."""
 import sys
 
 import numpy as np
 import pytest
 
-from nodebleed_amd import Scene, check, lib
+from nodebleed_amd import HIT, Scene, check, lib
 from oracle_lib import or_occluded, or_trace
 from raygen import mismatches, random_rays
 
@@ -76,3 +77,39 @@ def test_eight_million_rays(scene):
     assert (got["t"][hit] > 0).all() and (got["t"][hit] < rays["tmax"][hit]).all()
     assert np.array_equal(got["t"][~hit].view(np.uint32), rays["tmax"][~hit].view(np.uint32))
     assert ((got["id"][hit] & 0xFFF) < scene.view().instcnt).all()
+
+
+
+def test_iteration_limit_is_reported_then_cleared(scene):
+    """With a 64-step limit a ray that needs more steps (node, leaf and
+    instance visits, the oracle's per-ray counts) ends early and the call
+    reports REND_EOVERFLOW ("iteration limit") naming it, for closest hit
+    and, with a ray that misses everything, any hit; the report clears the
+    error, and with the default limit the rays trace to the oracle's
+    results again."""
+    L = lib()
+    rays = random_rays(scene, 4096, seed=7)
+    want, _, per = or_trace(scene, rays, per_ray=True)
+    steps = per.astype(np.int64).sum(axis=1)
+    i = int(np.argmax(steps))
+    miss = np.flatnonzero(want["id"] == MISS)
+    j = int(miss[np.argmax(steps[miss])]) if len(miss) else i
+    say(f"edges: iteration limit 64; closest-hit ray {i} takes {steps[i]} steps, "
+        f"missing ray {j} {steps[j]}")
+    assert steps[i] > 64 and steps[j] > 64
+    hits = np.zeros(1, HIT)
+    occ = np.zeros(1, np.uint8)
+    try:
+        check(L.rend_hip_set_iter_limit(64), "limit")
+        one = np.ascontiguousarray(rays[i:i + 1])
+        rc = L.rend_hip_trace_host(scene.rd, one.ctypes.data, hits.ctypes.data, 1, None)
+        msg = L.rend_hip_errstr().decode()
+        assert rc == -5 and "iteration limit" in msg and "ray 0 " in msg, (rc, msg)
+        one = np.ascontiguousarray(rays[j:j + 1])
+        rc = L.rend_hip_occluded_host(scene.rd, one.ctypes.data, occ.ctypes.data, 1)
+        msg = L.rend_hip_errstr().decode()
+        assert rc == -5 and "iteration limit" in msg, (rc, msg)
+    finally:
+        check(L.rend_hip_set_iter_limit(4194304), "limit")
+    assert mismatches(scene.trace(rays), want) == {"id": 0, "t": 0, "u": 0, "v": 0}
+    assert np.array_equal(scene.occluded(rays), or_occluded(scene, rays))
