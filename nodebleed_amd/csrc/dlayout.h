@@ -82,10 +82,15 @@ struct CamArgs {
  * one per XCD (workgroups are dispatched to the 8 XCDs round-robin), each
  * with its own claim counter on a separate 128-byte line; waves claim
  * RK_CHUNK rays at a time from their XCD's segment and steal from the other
- * segments once it is drained.  The error word follows. */
+ * segments once it is drained.  The error word follows.  A wave's lanes hold
+ * rays from within one chunk (two refills of 48 lanes), so the chunk also
+ * sets how coherent a wave's rays are when the ray order is (scanline
+ * primaries, path queues in pixel order); measured over 64..256 it is best
+ * at 96: 1080p terrain diffuse rays 361 -> 352 us against 256, instanced
+ * primaries 1139 -> 872 us, instanced path-traced frames 411 -> 291 ms. */
 #define RK_NSEG        8
 #define RK_SEG_STRIDE  32
-#define RK_CHUNK       256
+#define RK_CHUNK       96
 #define RK_CTL_ERR     (RK_NSEG * RK_SEG_STRIDE)
 #define RK_CTL_WORDS   (RK_CTL_ERR + RK_SEG_STRIDE)
 
