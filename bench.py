@@ -403,6 +403,11 @@ def guide_equal_error(a):
     from the clock).  Returns the result record; ratio = T_g / T_p."""
     from nodebleed_amd import REND_MODE_PATHTRACE
     L = lib()
+    # Guided frames launch their kernels directly (the network changes
+    # between frames); plain frames would replay a captured graph, which at
+    # this frame size is ~3 % slower than direct launches.  Both are timed
+    # with direct launches, so plain runs at its faster setting.
+    check(L.rend_hip_pt_graphs(0), "graphs")
     w, h = (W, H) if not a.guide_small else (480, 270)
     spf = a.guide_spf
     scene = Scene(config=5, seed=a.seed, detail=1.0, width=w, height=h, spp=spf)
@@ -485,7 +490,9 @@ def guide_equal_error(a):
         f" plain's {err_p:.4f}); ratio {ratio:.3f}")
     L.rend_hip_guide_disable(scene.rd)
     scene.close()
+    check(L.rend_hip_pt_graphs(0 if a.no_graphs else 1), "graphs")
     return {"ratio": round(ratio, 4) if reached else None,
+            "launch": "direct kernel launches for plain and guided frames",
             "workload": f"occluded-lights terrain (1M tris, 8 upward emitters capping chimneys "
                         f"light ceiling spots, terrain lit indirectly), {w}x{h}, {spf} spp/frame, "
                         f"max depth {maxdepth}, RR after {rr}; reference "
